@@ -1,0 +1,88 @@
+"""Reader/runner for oracle/_ref/ref_mesh_dump (reference mesh.cpp built in place).
+
+Test infrastructure: the dumper is built by oracle/Makefile only where the
+reference tree exists; tests that need it skip when it is absent.
+"""
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DUMPER = os.path.join(ROOT, "oracle", "_ref", "ref_mesh_dump")
+
+
+def available():
+    return os.path.exists(DUMPER)
+
+
+def _read(path):
+    raw = open(path, "rb").read()
+    off = 0
+
+    def take(n, dt):
+        nonlocal off
+        a = np.frombuffer(raw, dtype=dt, count=n, offset=off).copy()
+        off += n * np.dtype(dt).itemsize
+        return a
+
+    n1, nelt, npts, tfp = take(4, np.int32)
+    out = {"n1": int(n1), "nelt": int(nelt), "npts": int(npts), "tfp": int(tfp)}
+    out["z"] = take(n1, np.float64)
+    out["w"] = take(n1, np.float64)
+    out["D"] = take(n1 * n1, np.float64)
+    out["Dt"] = take(n1 * n1, np.float64)
+    out["w3"] = take(n1 ** 3, np.float64)
+    out["corners"] = take(nelt * 24, np.float64)
+    for k in ("x", "y", "zc", "jac", "rx", "sx", "tx", "ry", "sy", "ty", "rz", "sz", "tz"):
+        out[k] = take(npts, np.float64)
+    for k in ("vM", "vP", "bc"):
+        out[k] = take(tfp, np.int32)
+    for k in ("nx", "ny", "nz", "dA"):
+        out[k] = take(tfp, np.float64)
+    out["dxmin"], out["auto_cfl"], out["auto_dt"] = take(3, np.float64)
+    assert off == len(raw)
+    return out
+
+
+def box(tmpdir, n1, nel, bounds, periodic=(0, 0, 0)):
+    path = os.path.join(str(tmpdir), "box_%d.bin" % n1)
+    args = [DUMPER, "box", str(n1)] + [str(v) for v in nel] + [repr(float(b)) for b in bounds] \
+        + [str(int(p)) for p in periodic] + [path]
+    subprocess.run(args, check=True, stdout=subprocess.DEVNULL)
+    return _read(path)
+
+
+BC_NAMES = {0: "E", 1: "PERIODIC", 2: "PEC", 3: "PMC"}
+
+
+def write_rea(path, corners, kind, nbr_e, nbr_f):
+    """NekCEM .rea text in the layout Mesh::loadFromRea parses (mesh.cpp:317-445)."""
+    corners = np.asarray(corners, dtype=np.float64).reshape(-1, 8, 3)
+    nelt = corners.shape[0]
+    lines = ["generated mesh", "**MESH DATA**", "%d 3 %d NEL,NDIM,NELV" % (nelt, nelt)]
+    for e in range(nelt):
+        lines.append("ELEMENT %d [ 1a] GROUP 0" % (e + 1))
+        c = corners[e]
+        for lo in (0, 4):
+            for d in range(3):
+                lines.append(" ".join(repr(float(c[lo + a, d])) for a in range(4)))
+    lines.append("***** CURVED SIDE DATA *****")
+    lines.append("***** BOUNDARY CONDITIONS *****")
+    lines.append("***** FLUID BOUNDARY CONDITIONS *****")
+    kind = np.asarray(kind).reshape(-1)
+    for fi in range(nelt * 6):
+        e, f = divmod(fi, 6)
+        ne = int(nbr_e[fi]) + 1 if nbr_e[fi] >= 0 else 0
+        nf = int(nbr_f[fi]) + 1 if nbr_f[fi] >= 0 else 0
+        lines.append("%s %d %d %d.0 %d.0" % (BC_NAMES[int(kind[fi])], e + 1, f + 1, ne, nf))
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+
+
+def rea(tmpdir, n1, corners, kind, nbr_e, nbr_f, name="mesh"):
+    rp = os.path.join(str(tmpdir), name + ".rea")
+    bp = os.path.join(str(tmpdir), name + ".bin")
+    write_rea(rp, corners, kind, nbr_e, nbr_f)
+    subprocess.run([DUMPER, "rea", str(n1), rp, bp], check=True, stdout=subprocess.DEVNULL)
+    return _read(bp)
