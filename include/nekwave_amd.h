@@ -1,0 +1,203 @@
+/*
+ * nekwave_amd.h -- C ABI of the MI355X-native DG-SEM Maxwell step.
+ *
+ * Drop-in boundary for NekWave's solver path.  The reference exposes it as the
+ * C++ class DgSolver (src/dg_solver.hpp:35-102) fed by a Mesh
+ * (src/mesh.hpp:51-218), with the device work behind nekwave::device::launch_*
+ * (src/device/dg_kernels.hpp:13-88).  Each entry point below names the
+ * reference interface it replaces.  Plain pointers and sizes only; every call
+ * returns an NW_* status (never aborts) and nw_last_error() gives the text.
+ *
+ * Threading: one nw_dg per device, externally synchronized (the reference's
+ * single host thread + single stream, dg_solver.cpp:114).  The library owns all
+ * device memory, its stream and its graphs; the caller owns host arrays and no
+ * host pointer is retained after a call returns.
+ */
+#ifndef NEKWAVE_AMD_H
+#define NEKWAVE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NW_ABI_VERSION 1
+
+/* status codes */
+enum {
+    NW_OK = 0,
+    NW_ERR_ARG = 1,         /* bad argument (the reference asserts, dg_solver.cpp:244,265,308) */
+    NW_ERR_HIP = 2,         /* HIP runtime error (the reference aborts, gpu_runtime.hpp:113-121) */
+    NW_ERR_ALLOC = 3,       /* device allocation failed */
+    NW_ERR_STATE = 4,       /* call out of order (e.g. step before upload) */
+    NW_ERR_MESH = 5,        /* invalid mesh (inverted element, non-conforming face) */
+    NW_ERR_UNSUPPORTED = 6  /* n1/precision/pipeline combination not built */
+};
+
+/* arithmetic of the stage (fields / LSRK register) */
+enum { NW_PREC_FP64 = 0, NW_PREC_FP32 = 1, NW_PREC_FP32_K64 = 2 };
+/* NW_PIPE_PLAIN: the reference's 7-kernel stage (dg_solver.cpp:393-505), FP64 only.
+ * NW_PIPE_FUSED: one fused kernel per stage. */
+enum { NW_PIPE_PLAIN = 0, NW_PIPE_FUSED = 1 };
+/* face kinds: the bcType strings of FaceInfo (mesh.hpp:15-21, mesh.cpp:730-830) */
+enum { NW_FACE_E = 0, NW_FACE_PERIODIC = 1, NW_FACE_PEC = 2, NW_FACE_PMC = 3 };
+
+const char* nw_last_error(void);
+int nw_abi_version(void);
+/* Number of visible devices and properties of `device` (name may be NULL). */
+int nw_device_info(int device, int* ndev, char* name, size_t name_len, int* cus, size_t* mem_bytes);
+
+/* ------------------------------------------------------------------------- */
+/* Mesh -- replaces Mesh (src/mesh.hpp:51-218)                               */
+/* ------------------------------------------------------------------------- */
+typedef struct nw_mesh nw_mesh;
+
+typedef struct nw_mesh_info {
+    int n1;          /* GLL points per direction (Mesh::getN) */
+    int np;          /* n1^3 (Mesh::getNumPointsPerElement) */
+    int nelt;        /* Mesh::getNumElements */
+    int nhalo;       /* ghost trace slots after npts (Mesh::getNumHaloPoints) */
+    int64_t npts;    /* Mesh::getTotalPoints */
+    int64_t tfp;     /* total face points, 6 n1^2 nelt */
+    int has_media;   /* per-element eps/mu set */
+    int all_affine;  /* every element is a parallelepiped */
+} nw_mesh_info;
+
+/* Mesh::setN + Mesh::createBoxMesh (mesh.cpp:56-61, 683-843): element order
+ * ex + nelx (ey + nely ez), corner order mesh.cpp:721-728, faces PEC/PERIODIC/E.
+ * bounds = {xmin,xmax,ymin,ymax,zmin,zmax}.  Extension: jitter > 0 moves every
+ * vertex whose three indices are interior by U[-jitter h, +jitter h] per
+ * coordinate (mt19937_64(seed), uniform = (x >> 11) 2^-53, vertices x-fastest,
+ * draws x,y,z). */
+int nw_mesh_box(int n1, int nelx, int nely, int nelz, const double bounds[6], int periodic_x, int periodic_y,
+                int periodic_z, double jitter, uint64_t seed, nw_mesh** out);
+/* The mesh Mesh::loadFromRea produces (mesh.cpp:317-445): corners [nelt][8][3]
+ * and the FaceInfo list (kind, neighbour element, neighbour face; -1 = none).
+ * E faces are matched by node coordinates like setupFaceData (mesh.cpp:1048-1070)
+ * and must be conforming (one of the 8 square symmetries). */
+int nw_mesh_from_corners(int n1, int nelt, const double* corners, const int* face_kind, const int* nbr_elem,
+                         const int* nbr_face, nw_mesh** out);
+/* Extension: per-element permittivity/permeability (NULL = 1).  The inverse
+ * mass uses them (dg_kernels.cu:365-370 has eps = mu = 1) and the upwind flux
+ * becomes impedance weighted; eps = mu = 1 reproduces the reference. */
+int nw_mesh_set_materials(nw_mesh* m, const double* eps, const double* mu);
+/* Extension: single-GPU share of a partitioned domain (Mesh::partition,
+ * mesh.cpp:1083-1406): every x-periodic face becomes a cut face whose plus
+ * trace is a ghost slot after npts, filled each stage by the on-device
+ * pack -> copy -> unpack that stands in for the MPI exchange
+ * (dg_solver.cpp:334-441).  Returns the number of ghost trace slots. */
+int nw_mesh_self_halo_x(nw_mesh* m, int* nhalo);
+int nw_mesh_get_info(const nw_mesh* m, nw_mesh_info* out);
+/* getGllZ/getGllW (n1), getD (column-major n1*n1, D[i + j n1] = D_ij), getW3 (n1^3). */
+int nw_mesh_gll(const nw_mesh* m, double* z, double* w, double* D, double* w3);
+/* getCoordX/Y/Z (npts each), computed on the host. */
+int nw_mesh_coords(const nw_mesh* m, double* x, double* y, double* z);
+/* Element corners [nelt][8][3] (getElementCorners). */
+int nw_mesh_corners(const nw_mesh* m, double* corners);
+/* computeMinNodeDistance (mesh.cpp:883-930). */
+int nw_mesh_min_node_distance(const nw_mesh* m, double* out);
+/* computeAutomaticDt (mesh.cpp:947-951): cfl(safety) * dxmin / speed. */
+int nw_mesh_auto_dt(const nw_mesh* m, double speed, double safety, double* out);
+void nw_mesh_destroy(nw_mesh* m);
+
+/* ------------------------------------------------------------------------- */
+/* Solver -- replaces DgSolver (src/dg_solver.hpp:35-102)                    */
+/* ------------------------------------------------------------------------- */
+typedef struct nw_dg nw_dg;
+
+typedef struct nw_opts {
+    int precision;    /* NW_PREC_* (default FP64) */
+    int pipeline;     /* NW_PIPE_* (default FUSED) */
+    int use_graph;    /* capture each step as a hipGraph (default 1) */
+    int device;       /* HIP device ordinal (default 0) */
+    int tile_option;  /* fused elements-per-workgroup option 0/1, -1 = tuned default */
+} nw_opts;
+void nw_opts_default(nw_opts* o);
+
+/* DgSolver::initialize (dg_solver.cpp:40-241): all geometry to the device.  c0 is
+ * the flux penalty (0 central, 1 upwind).  The mesh is only read during the call. */
+int nw_dg_create(const nw_mesh* m, double c0, const nw_opts* opts, nw_dg** out);
+/* DgSolver::uploadState (dg_solver.cpp:243-262): host [Ex|Ey|Ez|Hx|Hy|Hz], n = 6 npts. */
+int nw_dg_upload(nw_dg* s, const double* host, size_t n);
+/* DgSolver::downloadState (dg_solver.cpp:264-283). */
+int nw_dg_download(nw_dg* s, double* host, size_t n);
+/* DgSolver::step (dg_solver.cpp:306-527): one LSRK45 step, synchronous like the
+ * reference.  `time` is unused, as in the reference (dg_solver.cpp:307). */
+int nw_dg_step(nw_dg* s, double dt, double time);
+/* nsteps steps with one synchronization at the end. */
+int nw_dg_run(nw_dg* s, double dt, int nsteps);
+/* nsteps steps timed with HIP events on the solver's stream (elapsed_ms). */
+int nw_dg_time(nw_dg* s, double dt, int nsteps, double* elapsed_ms);
+int nw_dg_sync(nw_dg* s);
+
+typedef struct nw_diag {
+    double energy;   /* 1/2 sum (eps|E|^2 + mu|H|^2) J w_i w_j w_k, FP64 (case.cpp:733-766) */
+    double max_e;    /* max |E| (case.cpp:768-780) */
+    double max_h;    /* max |H| (case.cpp:782-794) */
+} nw_diag;
+/* Diagnostics reduced on the device (no state download). */
+int nw_dg_diag(nw_dg* s, nw_diag* out);
+
+typedef struct nw_dg_info {
+    int n1, nelt, precision, pipeline, use_graph, epw;
+    int64_t npts, stride, device_bytes;
+    int kernels_per_step;
+} nw_dg_info;
+int nw_dg_get_info(const nw_dg* s, nw_dg_info* out);
+/* DgSolver::finalize (dg_solver.cpp:529-582) + destructor. */
+void nw_dg_destroy(nw_dg* s);
+
+/* ------------------------------------------------------------------------- */
+/* Kernel level -- replaces nekwave::device::launch_* (dg_kernels.hpp:13-88) */
+/* Device pointers; stream NULL = default stream.                            */
+/* ------------------------------------------------------------------------- */
+/* launch_volume_curl (dg_kernels.hpp:13-21; kernel dg_kernels.cu:41-155). */
+int nw_launch_volume_curl(const double* u1, const double* u2, const double* u3, double* w1, double* w2, double* w3,
+                          const double* D, const double* w3_gll, const double* jac, const double* rx,
+                          const double* sx, const double* tx, const double* ry, const double* sy, const double* ty,
+                          const double* rz, const double* sz, const double* tz, double sign, int n1, int nelt,
+                          void* stream);
+/* launch_restrict_faces (dg_kernels.hpp:26-30; dg_kernels.cu:170-190). */
+int nw_launch_restrict_faces(const double* state, double* fEN, double* fHN, const int* volIdxMinus,
+                             int64_t totalFacePoints, int64_t stateStride, void* stream);
+/* launch_compute_flux (dg_kernels.hpp:35-41; dg_kernels.cu:220-304).  Extension:
+ * eps/mu per element (NULL = reference flux), with volIdxMinus, ghost_elem (the
+ * element owning each ghost slot) and np to find both sides' media. */
+int nw_launch_compute_flux(const double* state, const double* fEN, const double* fHN, const int* volIdxPlus,
+                           const int* isPEC, const double* nx, const double* ny, const double* nz, double* flux,
+                           double c0, int64_t totalFacePoints, int64_t stateStride, const int* volIdxMinus,
+                           const double* eps, const double* mu, const int* ghost_elem, int np, int64_t npts,
+                           void* stream);
+/* launch_add_flux (dg_kernels.hpp:46-50; dg_kernels.cu:316-338).  Deterministic:
+ * each node gathers its face points in face order; requires the reference's
+ * element-major face-point layout (dg_solver.cpp:76-112), so volIdxMinus is
+ * implied and n1 is needed instead. */
+int nw_launch_add_flux(const double* flux, const double* dA, double* rhs, int64_t totalFacePoints, int64_t npts,
+                       int n1, void* stream);
+/* launch_inv_mass (dg_kernels.hpp:55-59; dg_kernels.cu:353-382); eps/mu per element or NULL. */
+int nw_launch_inv_mass(double* rhs, const double* jac, const double* w3, int nelt, int nxyz, int64_t npts,
+                       const double* eps, const double* mu, void* stream);
+/* launch_lsrk45_update (dg_kernels.hpp:64-68; dg_kernels.cu:393-408). */
+int nw_launch_lsrk45_update(double* state, double* k, const double* rhs, double rk4a, double rk4b, double dt,
+                            int64_t npts, int64_t stateStride, void* stream);
+/* launch_pack_halo / launch_unpack_halo (dg_kernels.hpp:73-90; dg_kernels.cu:413-453). */
+int nw_launch_pack_halo(const double* state, double* sendBuf, const int* sendVolIndices, const int* sendBufOffsets,
+                        const int* exchangeSizes, int numHaloPoints, int64_t stateStride, void* stream);
+int nw_launch_unpack_halo(const double* recvBuf, double* state, const int* recvBufOffsets, const int* exchangeSizes,
+                          int numHaloPoints, int64_t npts, int64_t stateStride, void* stream);
+
+/* Device memory helpers for callers without their own allocator (tests, FFI). */
+int nw_dev_alloc(size_t bytes, void** p);
+int nw_dev_free(void* p);
+int nw_dev_h2d(void* dst, const void* src, size_t bytes);
+int nw_dev_d2h(void* dst, const void* src, size_t bytes);
+int nw_dev_sync(void);
+/* Plain device-to-device copy kernel of `bytes` (HBM calibration), ms per copy. */
+int nw_bench_copy(size_t bytes, int reps, double* ms_per_copy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NEKWAVE_AMD_H */

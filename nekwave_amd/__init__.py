@@ -1,0 +1,306 @@
+"""nekwave_amd -- MI355X-native DG-SEM time-domain Maxwell step.
+
+Python mirror of the reference's host interface over the library's C ABI
+(include/nekwave_amd.h):
+
+    Mesh       -- reference Mesh (src/mesh.hpp:51-218): createBoxMesh, GLL data,
+                  coordinates, dt helpers; extensions for jitter, media and the
+                  single-GPU halo stand-in.
+    DgSolver   -- reference DgSolver (src/dg_solver.hpp:35-102): initialize,
+                  uploadState, downloadState, step, finalize; extensions for
+                  precision, pipeline, graphs, timed runs and device diagnostics.
+    Case       -- reference Case lifecycle (src/case.hpp:48-153):
+                  preprocess / simulate / postprocess over a Config subset.
+
+All numerics run in the HIP library; nothing here computes the step.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import FACE_E, FACE_PEC, FACE_PERIODIC, FACE_PMC, NekWaveError, check, lib
+
+__all__ = ["Mesh", "DgSolver", "Case", "Config", "DeviceBuffer", "NekWaveError", "device_info",
+           "FACE_E", "FACE_PERIODIC", "FACE_PEC", "FACE_PMC"]
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def device_info(device=0):
+    n = C.c_int(0)
+    name = C.create_string_buffer(64)
+    cus = C.c_int(0)
+    mem = C.c_size_t(0)
+    check(lib().nw_device_info(device, C.byref(n), name, 64, C.byref(cus), C.byref(mem)))
+    return {"count": n.value, "arch": name.value.decode(), "cus": cus.value, "mem_bytes": mem.value}
+
+
+class Mesh:
+    """Reference Mesh: construct with the number of GLL points per direction N
+    (the reference's `order`), then build with createBoxMesh (or fromCorners)."""
+
+    def __init__(self, N=3, numElements=1):
+        self._N = int(N)
+        self._h = None
+        self._nel0 = int(numElements)
+
+    # -- construction -------------------------------------------------------
+    def setN(self, N):
+        if self._h is not None:
+            raise NekWaveError(_lib.NW_ERR_STATE, "setN after the mesh is built")
+        self._N = int(N)
+
+    def createBoxMesh(self, nelx, nely, nelz, xmin=-1.0, xmax=1.0, ymin=-1.0, ymax=1.0, zmin=-1.0, zmax=1.0,
+                      periodicX=False, periodicY=False, periodicZ=False, jitter=0.0, seed=0):
+        self._free()
+        b = np.array([xmin, xmax, ymin, ymax, zmin, zmax], dtype=np.float64)
+        h = C.c_void_p()
+        check(lib().nw_mesh_box(self._N, int(nelx), int(nely), int(nelz), _dp(b), int(bool(periodicX)),
+                                int(bool(periodicY)), int(bool(periodicZ)), float(jitter), int(seed), C.byref(h)))
+        self._h = h
+        self.box = (int(nelx), int(nely), int(nelz))
+        return True
+
+    @classmethod
+    def fromCorners(cls, N, corners, kind, nbr_elem, nbr_face):
+        m = cls(N)
+        corners = np.ascontiguousarray(corners, dtype=np.float64).reshape(-1)
+        kind = np.ascontiguousarray(kind, dtype=np.int32).reshape(-1)
+        ne = np.ascontiguousarray(nbr_elem, dtype=np.int32).reshape(-1)
+        nf = np.ascontiguousarray(nbr_face, dtype=np.int32).reshape(-1)
+        h = C.c_void_p()
+        check(lib().nw_mesh_from_corners(m._N, corners.size // 24, _dp(corners), _ip(kind), _ip(ne), _ip(nf),
+                                         C.byref(h)))
+        m._h = h
+        return m
+
+    def setMaterials(self, eps=None, mu=None):
+        e = None if eps is None else np.ascontiguousarray(eps, dtype=np.float64)
+        u = None if mu is None else np.ascontiguousarray(mu, dtype=np.float64)
+        for a in (e, u):
+            if a is not None and a.size != self.getNumElements():
+                raise NekWaveError(_lib.NW_ERR_ARG, "materials need one value per element")
+        check(lib().nw_mesh_set_materials(self.handle, _dp(e) if e is not None else None,
+                                          _dp(u) if u is not None else None))
+
+    def selfHaloX(self):
+        n = C.c_int(0)
+        check(lib().nw_mesh_self_halo_x(self.handle, C.byref(n)))
+        return n.value
+
+    # -- queries --------------------------------------------------------------
+    @property
+    def handle(self):
+        if self._h is None:
+            raise NekWaveError(_lib.NW_ERR_STATE, "mesh not built")
+        return self._h
+
+    def info(self):
+        o = _lib.MeshInfo()
+        check(lib().nw_mesh_get_info(self.handle, C.byref(o)))
+        return o
+
+    def getN(self):
+        return self._N
+
+    def getNumPointsPerElement(self):
+        return self._N ** 3
+
+    def getNumElements(self):
+        return self.info().nelt if self._h is not None else self._nel0
+
+    def getTotalPoints(self):
+        return self.info().npts
+
+    def getNumHaloPoints(self):
+        return self.info().nhalo
+
+    def isPartitioned(self):
+        return self.getNumHaloPoints() > 0
+
+    def _gll(self):
+        n = self._N
+        z = np.zeros(n); w = np.zeros(n); D = np.zeros(n * n); w3 = np.zeros(n ** 3)
+        check(lib().nw_mesh_gll(self.handle, _dp(z), _dp(w), _dp(D), _dp(w3)))
+        return z, w, D, w3
+
+    def getGllZ(self):
+        return self._gll()[0]
+
+    def getGllW(self):
+        return self._gll()[1]
+
+    def getD(self):
+        return self._gll()[2]
+
+    def getW3(self):
+        return self._gll()[3]
+
+    def coords(self):
+        n = self.getTotalPoints()
+        x = np.zeros(n); y = np.zeros(n); z = np.zeros(n)
+        check(lib().nw_mesh_coords(self.handle, _dp(x), _dp(y), _dp(z)))
+        return x, y, z
+
+    def getCoordX(self):
+        return self.coords()[0]
+
+    def getCoordY(self):
+        return self.coords()[1]
+
+    def getCoordZ(self):
+        return self.coords()[2]
+
+    def getElementCorners(self):
+        c = np.zeros(self.getNumElements() * 24)
+        check(lib().nw_mesh_corners(self.handle, _dp(c)))
+        return c.reshape(-1, 8, 3)
+
+    def computeMinNodeDistance(self):
+        v = C.c_double()
+        check(lib().nw_mesh_min_node_distance(self.handle, C.byref(v)))
+        return v.value
+
+    @staticmethod
+    def computeAutomaticCFL(safetyFactor=0.4):
+        return (safetyFactor * 1.75) / np.sqrt(3.0)
+
+    def computeAutomaticDt(self, waveSpeed=1.0, safetyFactor=0.4):
+        v = C.c_double()
+        check(lib().nw_mesh_auto_dt(self.handle, float(waveSpeed), float(safetyFactor), C.byref(v)))
+        return v.value
+
+    def _free(self):
+        if self._h is not None:
+            lib().nw_mesh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self._free()
+        except Exception:
+            pass
+
+
+class DgSolver:
+    """Reference DgSolver.  Extensions: precision ('fp64', 'fp32', 'fp32_k64'),
+    pipeline ('fused' or 'plain' = the reference's 7-kernel stage), graphs."""
+
+    def __init__(self, precision="fp64", pipeline="fused", use_graph=True, device=0, tile_option=-1):
+        self.opts = _lib.Opts()
+        lib().nw_opts_default(C.byref(self.opts))
+        self.opts.precision = _lib.PREC[precision]
+        self.opts.pipeline = _lib.PIPE[pipeline]
+        self.opts.use_graph = int(bool(use_graph))
+        self.opts.device = int(device)
+        self.opts.tile_option = int(tile_option)
+        self._h = None
+        self.npts = 0
+
+    def initialize(self, mesh, c0):
+        self.finalize()
+        h = C.c_void_p()
+        check(lib().nw_dg_create(mesh.handle, float(c0), C.byref(self.opts), C.byref(h)))
+        self._h = h
+        self.npts = mesh.getTotalPoints()
+        return True
+
+    def uploadState(self, hostState, size=None):
+        a = np.ascontiguousarray(hostState, dtype=np.float64).reshape(-1)
+        n = a.size if size is None else int(size)
+        check(lib().nw_dg_upload(self._handle(), _dp(a), n))
+
+    def downloadState(self, hostState=None, size=None):
+        n = 6 * self.npts if size is None else int(size)
+        out = np.zeros(n) if hostState is None else hostState
+        if out.dtype != np.float64 or not out.flags.c_contiguous:
+            raise NekWaveError(_lib.NW_ERR_ARG, "downloadState needs a contiguous float64 array")
+        check(lib().nw_dg_download(self._handle(), _dp(out), n))
+        return out
+
+    def step(self, dt, time=0.0):
+        check(lib().nw_dg_step(self._handle(), float(dt), float(time)))
+
+    def run(self, dt, nsteps):
+        check(lib().nw_dg_run(self._handle(), float(dt), int(nsteps)))
+
+    def time(self, dt, nsteps):
+        ms = C.c_double()
+        check(lib().nw_dg_time(self._handle(), float(dt), int(nsteps), C.byref(ms)))
+        return ms.value
+
+    def diagnostics(self):
+        o = _lib.Diag()
+        check(lib().nw_dg_diag(self._handle(), C.byref(o)))
+        return {"energy": o.energy, "max_e": o.max_e, "max_h": o.max_h}
+
+    def info(self):
+        o = _lib.DgInfo()
+        check(lib().nw_dg_get_info(self._handle(), C.byref(o)))
+        return {f: getattr(o, f) for f, _ in o._fields_}
+
+    def finalize(self):
+        if self._h is not None:
+            lib().nw_dg_destroy(self._h)
+            self._h = None
+
+    def _handle(self):
+        if self._h is None:
+            raise NekWaveError(_lib.NW_ERR_STATE, "solver not initialized")
+        return self._h
+
+    def __del__(self):
+        try:
+            self.finalize()
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    """A device allocation owned by the library's allocator (for the kernel-level ABI)."""
+
+    def __init__(self, nbytes):
+        self.ptr = C.c_void_p()
+        self.nbytes = int(nbytes)
+        check(lib().nw_dev_alloc(self.nbytes, C.byref(self.ptr)))
+
+    @classmethod
+    def of(cls, arr):
+        a = np.ascontiguousarray(arr)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        check(lib().nw_dev_h2d(self.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes))
+
+    def download(self, dtype, count):
+        out = np.zeros(count, dtype=dtype)
+        check(lib().nw_dev_d2h(out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes))
+        return out
+
+    def offset(self, nbytes):
+        return C.c_void_p(self.ptr.value + int(nbytes))
+
+    def free(self):
+        if self.ptr:
+            lib().nw_dev_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+from .case import Case, Config  # noqa: E402

@@ -1,0 +1,187 @@
+"""Case lifecycle over the solver (reference Case, src/case.hpp:48-153, case.cpp:94-728).
+
+preprocess(): box mesh, dt rule, initial condition hook evaluated at the nodes.
+simulate():   DgSolver.initialize + uploadState, then the step loop with the
+              reference's termination and last-dt clamp (case.cpp:562-583).
+              Unlike the reference (which downloads the whole state every step,
+              case.cpp:588) diagnostics are reduced on the device and the state
+              is downloaded once at the end.
+postprocess(): user hook, then DgSolver.finalize.
+
+Config keeps the subset of the reference's keys that shape the step
+(config.hpp:21-254): order, cfl, dt, c0/flux_type, nelx/nely/nelz, bounds,
+periodic_*, num_steps/max_steps, final_time, output_frequency; plus the
+extensions precision, pipeline, jitter, seed.
+"""
+import json
+import math
+
+import numpy as np
+
+from . import DgSolver, Mesh
+
+
+def _six_sig(v):
+    """The reference round-trips JSON numbers through a default-precision string
+    (json.hpp:52-61), keeping 6 significant digits."""
+    return float("%g" % float(v))
+
+
+class Config:
+    _ALIASES = {"n": "order", "polynomial_order": "order", "num_steps": "num_steps", "nsteps": "num_steps",
+                "max_steps": "num_steps", "t_final": "final_time", "final_time": "final_time",
+                "output_frequency": "output_frequency", "outputfreq": "output_frequency",
+                "flux": "flux_type"}
+
+    def __init__(self, **kw):
+        self.order = 3
+        self.cfl = -1.0
+        self.dt = -1.0
+        self.num_steps = 100
+        self.final_time = -1.0
+        self.output_frequency = 10
+        self.c0 = 1.0
+        self.nelx = self.nely = self.nelz = 0
+        self.xmin, self.xmax, self.ymin, self.ymax, self.zmin, self.zmax = -1.0, 1.0, -1.0, 1.0, -1.0, 1.0
+        self.periodic_x = self.periodic_y = self.periodic_z = False
+        self.precision = "fp64"
+        self.pipeline = "fused"
+        self.jitter = 0.0
+        self.seed = 0
+        for k, v in kw.items():
+            self.set(k, v)
+
+    def set(self, key, val):
+        k = self._ALIASES.get(key.lower(), key.lower())
+        if k == "flux_type":
+            self.c0 = 0.0 if str(val).lower() in ("central", "0", "0.0") else 1.0
+            return
+        if k in ("cfl", "dt") and isinstance(val, str) and val.lower() == "auto":
+            setattr(self, k, -1.0)
+            return
+        if not hasattr(self, k):
+            raise KeyError("unknown config key %r" % key)
+        cur = getattr(self, k)
+        if isinstance(cur, bool):
+            val = str(val).lower() in ("1", "true", "yes", "on") if not isinstance(val, bool) else val
+        elif isinstance(cur, int) and not isinstance(cur, bool):
+            val = int(val)
+        elif isinstance(cur, float):
+            val = float(val)
+        setattr(self, k, val)
+
+    @classmethod
+    def from_json(cls, path_or_dict):
+        """Flatten JSON sections like Config::loadFromJson (config.hpp:261-340)."""
+        d = path_or_dict if isinstance(path_or_dict, dict) else json.load(open(path_or_dict))
+        c = cls()
+        flat = {}
+        for k, v in d.items():
+            if isinstance(v, dict):
+                flat.update(v)
+            elif not k.startswith("_"):
+                flat[k] = v
+        for k, v in flat.items():
+            if isinstance(v, float):
+                v = _six_sig(v)
+            try:
+                c.set(k, v)
+            except KeyError:
+                pass
+        return c
+
+
+class Case:
+    def __init__(self, config=None):
+        self.config = config or Config()
+        self.ic = None
+        self.post = None
+        self.mesh = None
+        self.solver = None
+        self.state = None
+        self.dt = 0.0
+        self.cfl = 0.0
+        self.current_time = 0.0
+        self.current_step = 0
+        self.history = []
+
+    def setInitialConditionHook(self, fn):
+        """fn(x, y, z) -> (Ex, Ey, Ez, Hx, Hy, Hz), vectorised over node arrays."""
+        self.ic = fn
+
+    def setPostprocessingHook(self, fn):
+        self.post = fn
+
+    def preprocess(self):
+        c = self.config
+        self.mesh = Mesh(c.order)
+        if c.nelx > 0 and c.nely > 0 and c.nelz > 0:
+            self.mesh.createBoxMesh(c.nelx, c.nely, c.nelz, c.xmin, c.xmax, c.ymin, c.ymax, c.zmin, c.zmax,
+                                    c.periodic_x, c.periodic_y, c.periodic_z, jitter=c.jitter, seed=c.seed)
+        else:  # the reference's default 3x3x3 box (case.cpp:158-161)
+            self.mesh.createBoxMesh(3, 3, 3, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0)
+        dxmin = self.mesh.computeMinNodeDistance()
+        if c.dt > 0.0:  # case.cpp:201-210
+            self.dt = c.dt
+            self.cfl = self.dt / dxmin
+        elif c.cfl > 0.0:
+            self.cfl = c.cfl
+            self.dt = self.cfl * dxmin
+        else:
+            self.cfl = Mesh.computeAutomaticCFL(0.4)
+            self.dt = self.mesh.computeAutomaticDt(1.0, 0.4)
+        x, y, z = self.mesh.coords()
+        if self.ic is not None:
+            f = self.ic(x, y, z)
+            self.state = np.concatenate([np.broadcast_to(np.asarray(v, dtype=np.float64), x.shape) for v in f])
+        else:
+            self.state = np.zeros(6 * x.size)
+        self.current_time = 0.0
+        self.current_step = 0
+
+    def simulate(self):
+        c = self.config
+        self.solver = DgSolver(precision=c.precision, pipeline=c.pipeline)
+        self.solver.initialize(self.mesh, c.c0)
+        self.solver.uploadState(self.state)
+        max_steps, final_time = c.num_steps, c.final_time
+        out_freq = max(1, c.output_frequency)
+        step = 0
+        while True:  # case.cpp:562-583
+            if max_steps > 0 and step >= max_steps:
+                break
+            if final_time > 0.0 and self.current_time >= final_time - 1e-13:
+                break
+            dt = self.dt
+            if final_time > 0.0 and self.current_time + dt > final_time:
+                dt = final_time - self.current_time
+            step += 1
+            self.solver.step(dt, self.current_time)
+            self.current_time += dt
+            self.current_step = step
+            final = (max_steps > 0 and step >= max_steps) or (final_time > 0.0 and self.current_time >= final_time - 1e-13)
+            if step % out_freq == 0 or final:
+                d = self.solver.diagnostics()
+                self.history.append((step, self.current_time, d["max_e"], d["max_h"], d["energy"]))
+        self.state = self.solver.downloadState()
+
+    def postprocess(self):
+        if self.post is not None:
+            self.post(self)
+        if self.solver is not None:
+            self.solver.finalize()
+
+    def run(self):
+        self.preprocess()
+        self.simulate()
+        self.postprocess()
+
+    def computeTotalEnergy(self):
+        """Host energy of the downloaded state (case.cpp:733-766)."""
+        w3 = self.mesh.getW3()
+        m = self.mesh.info()
+        # jacobian from the product's geometry: J = dV / w3 per node via the
+        # solver's device reduction is the cheap path; here use diagnostics
+        if self.solver is not None and self.solver._h is not None:
+            return self.solver.diagnostics()["energy"]
+        raise RuntimeError("computeTotalEnergy needs a live solver (%d elements, %d weights)" % (m.nelt, w3.size))

@@ -1,0 +1,65 @@
+// nw_fused.hpp -- launch interface of the fused per-element LSRK45 stage kernel.
+#pragma once
+
+#include "nw_internal.hpp"
+
+namespace nw {
+
+enum Prec { kFP64 = 0, kFP32 = 1, kFP32K64 = 2, kNumPrec = 3 };
+
+// Everything one launch of the fused stage kernel needs.  One launch covers a
+// contiguous range of elements that share n1 (a whole mesh, or one n1 segment
+// of a batched set of independent meshes).
+struct FusedArgs {
+    const void* uin;          // 6 component blocks of R, block stride `stride`
+    void* uout;               // same layout
+    void* kbuf;               // 6 component blocks of KR, block stride `kstride`
+    long long stride;         // npts_total + nghost (ghost slots after the local nodes)
+    long long kstride;
+    long long node_base;      // first node of this segment
+    long long ghost_base;     // first ghost slot (node index) of this segment
+    int nelt;                 // elements in this segment
+    int nchunk;               // blocks per XCD chunk (XCD-aware block remap)
+    const ElemGeom* geom;     // per element, segment-local
+    const unsigned char* affine;
+    const ElemMat* mat;       // per element (always present; vacuum = 1)
+    const ElemMat* gmat;      // per ghost face
+    const int* fcode;         // per element face (nelt*6)
+    const double* dt;         // device scalar
+    double a, b, c0;
+    int first_stage;          // a == 0: k is not read
+    int weighted;             // impedance-weighted flux (media present)
+};
+
+// Elements per workgroup options (index 0/1) for a given n1.
+int fused_epw(int n1, int option, int prec);
+bool fused_supported(int n1, int prec);
+hipError_t launch_fused(int n1, int prec, int option, const FusedArgs& a, hipStream_t s);
+int fused_upload_tables(const GllTables& t);
+
+// Face orientation: (p, q) on a face -> (p', q') on the neighbour face.
+// bit0: swap p,q; bit1: p' = n1-1-p'; bit2: q' = n1-1-q'.
+__host__ __device__ inline void orient_map(int o, int n1, int p, int q, int& p2, int& q2) {
+    int a = (o & 1) ? q : p, b = (o & 1) ? p : q;
+    if (o & 2) a = n1 - 1 - a;
+    if (o & 4) b = n1 - 1 - b;
+    p2 = a;
+    q2 = b;
+}
+
+// getFaceNodeVolIndex (mesh.cpp:955-981) as local (i, j, k).
+__host__ __device__ inline void face_ijk(int f, int n1, int p, int q, int& i, int& j, int& k) {
+    switch (f) {
+        case 0: i = p; j = 0; k = q; break;
+        case 1: i = n1 - 1; j = p; k = q; break;
+        case 2: i = p; j = n1 - 1; k = q; break;
+        case 3: i = 0; j = p; k = q; break;
+        case 4: i = p; j = q; k = 0; break;
+        default: i = p; j = q; k = n1 - 1; break;
+    }
+}
+
+// fcode encoding: >= 0 : nbr_elem * 64 + nbr_face * 8 + orient; -1 : mirror; <= -2 : ghost face -(c+2)
+__host__ __device__ inline int fcode_pack(int e, int f, int o) { return e * 64 + f * 8 + o; }
+
+}  // namespace nw

@@ -1,0 +1,48 @@
+// nw_mesh.hpp -- host mesh description (the data DgSolver::initialize consumes).
+#pragma once
+
+#include <vector>
+
+#include "nw_internal.hpp"
+
+namespace nw {
+
+constexpr int NW_FACE_HALO_INTERNAL = 4;
+
+struct Mesh {
+    int n1 = 0, np = 0, nelt = 0;
+    int box_nel[3] = {0, 0, 0};
+    bool box_conforming = false;
+    bool all_affine = false;
+    bool media = false;
+    std::vector<double> corners;                 // nelt * 24
+    std::vector<int> kind, nbr_e, nbr_f;         // nelt * 6 (FaceInfo list)
+    std::vector<int> fcode, fbc;                 // nelt * 6 (neighbour code, reference bc code)
+    std::vector<ElemGeom> geom;
+    std::vector<unsigned char> affine;
+    std::vector<ElemMat> mat;
+    std::vector<double> eps, mu_;                // per element when media
+    int nghost_faces = 0;
+    std::vector<int> ghost_elem;                 // per ghost face: element owning the partner face
+    std::vector<ElemMat> gmat;                   // per ghost face
+    std::vector<int> send_vol;                   // per ghost slot: node the pack copies
+
+    static Mesh* box(int n1, int nelx, int nely, int nelz, const double* bounds, int px, int py, int pz,
+                     double jitter, uint64_t seed, int* status);
+    static Mesh* from_corners(int n1, int nelt, const double* corners, const int* kind, const int* ne,
+                              const int* nf, int* status);
+    int finish();
+    int set_materials(const double* eps, const double* mu);
+    int self_halo_x();
+    long long npts() const { return (long long)nelt * np; }
+    long long tfp() const { return (long long)nelt * 6 * n1 * n1; }
+    long long nhalo_points() const;
+    void coords(double* x, double* y, double* z) const;
+    double min_node_distance() const;
+};
+
+}  // namespace nw
+
+struct nw_mesh {
+    nw::Mesh m;
+};

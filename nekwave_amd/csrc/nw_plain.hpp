@@ -1,0 +1,53 @@
+// nw_plain.hpp -- launchers of the reference-decomposed FP64 kernels and the
+// device-side helpers (geometry setup, diagnostics, conversion, halo).
+#pragma once
+
+#include "nw_internal.hpp"
+
+namespace nw {
+
+struct PlainVolumeArgs {
+    const double *u1, *u2, *u3;
+    double *w1, *w2, *w3;
+    const double *D, *w3g, *jac;
+    const double *rx, *sx, *tx, *ry, *sy, *ty, *rz, *sz, *tz;
+    double sign;
+    int n1, nelt;
+};
+
+int plain_upload_tables(const GllTables& t);
+
+hipError_t launch_volume_curl(const PlainVolumeArgs& a, hipStream_t s);
+hipError_t launch_restrict(const double* st, double* fEN, double* fHN, const int* vM, long long tfp, long long S,
+                           hipStream_t s);
+hipError_t launch_flux(const double* st, const double* fEN, const double* fHN, const int* vP, const int* bc,
+                       const double* nx, const double* ny, const double* nz, double* flux, double c0, long long tfp,
+                       long long S, const int* vM, const double* eps, const double* mu, const int* ghost_elem, int np,
+                       long long npts, hipStream_t s);
+hipError_t launch_lift(const double* flux, const double* dA, double* rhs, long long tfp, long long npts, int n1,
+                       hipStream_t s);
+hipError_t launch_inv_mass(double* rhs, const double* jac, const double* w3, int nxyz, long long npts,
+                           const double* eps, const double* mu, hipStream_t s);
+hipError_t launch_lsrk(double* st, double* k, const double* rhs, double a, double b, const double* dtp, double dt,
+                       long long npts, long long S, hipStream_t s);
+template <typename R>
+hipError_t launch_pack(const R* st, R* buf, const int* vol, const int* off, const int* sizes, int nh, long long S,
+                       hipStream_t s);
+template <typename R>
+hipError_t launch_unpack(const R* buf, R* st, const int* off, const int* sizes, int nh, long long npts, long long S,
+                         hipStream_t s);
+hipError_t launch_metrics(const double* corners, int n1, long long npts, double* const* out13, hipStream_t s);
+hipError_t launch_face_data(const int* fcode, const int* fbc, int n1, long long tfp, long long npts,
+                            const double* const* met10, int* vM, int* vP, int* bc, double* nx, double* ny, double* nz,
+                            double* dA, hipStream_t s);
+int diag_blocks();
+template <typename R>
+hipError_t launch_diag(const R* u, long long S, long long npts, int n1, const double* jac, const ElemGeom* geom,
+                       const ElemMat* mat, double* part, double* out, hipStream_t s);
+template <typename D, typename S>
+hipError_t launch_convert(D* dst, long long dstride, const S* src, long long sstride, long long n, int ncomp,
+                          hipStream_t s);
+hipError_t launch_copy(const void* src, void* dst, long long bytes, hipStream_t s);
+hipError_t launch_fill(double* p, long long n, double v, hipStream_t s);
+
+}  // namespace nw

@@ -1,0 +1,430 @@
+// nw_solver.cpp -- the DgSolver replacement: device residency, stage
+// sequencing, graphs, diagnostics (dg_solver.cpp:40-582 is the reference).
+#include "nw_solver.hpp"
+
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "nw_fused.hpp"
+#include "nw_plain.hpp"
+
+namespace nw {
+
+namespace {
+std::mutex g_tables_mu;
+bool g_tables_done[64] = {false};
+}  // namespace
+
+int upload_gll_tables(int device) {
+    std::lock_guard<std::mutex> lk(g_tables_mu);
+    if (device < 0 || device >= 64) return NW_ERR_ARG;
+    if (g_tables_done[device]) return NW_OK;
+    const GllTables& t = host_gll_tables();
+    int st = fused_upload_tables(t);
+    if (st != NW_OK) return st;
+    st = plain_upload_tables(t);
+    if (st != NW_OK) return st;
+    g_tables_done[device] = true;
+    return NW_OK;
+}
+
+// LSRK45 coefficients, Carpenter & Kennedy (1994) (dg_solver.cpp:58-69).
+static const double kRkA[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                               -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+static const double kRkB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                               1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                               2277821191437.0 / 14882151754819.0};
+
+Solver::~Solver() { release(); }
+
+void Solver::release() {
+    if (device < 0) return;
+    (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (int p = 0; p < 2; ++p) {
+        if (gexec[p]) (void)hipGraphExecDestroy(gexec[p]);
+        if (graph[p]) (void)hipGraphDestroy(graph[p]);
+        gexec[p] = nullptr;
+        graph[p] = nullptr;
+    }
+    for (void* p : allocs) (void)hipFree(p);
+    allocs.clear();
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    ev0 = ev1 = nullptr;
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+    device = -1;
+}
+
+int Solver::dalloc(void** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        set_error("hipMalloc(" + std::to_string(bytes) + " B) failed: " + hipGetErrorString(e));
+        *p = nullptr;
+        return NW_ERR_ALLOC;
+    }
+    allocs.push_back(*p);
+    device_bytes += (long long)bytes;
+    return NW_OK;
+}
+
+template <typename T>
+int Solver::upload_vec(T** dst, const std::vector<T>& v) {
+    int st = dalloc(reinterpret_cast<void**>(dst), v.size() * sizeof(T));
+    if (st) return st;
+    if (!v.empty()) NW_TRY_HIP(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return NW_OK;
+}
+
+int Solver::init(const Mesh& m, double c0_, const nw_opts& o) {
+    NW_CHECK_ARG(o.precision >= 0 && o.precision < kNumPrec, "bad precision");
+    NW_CHECK_ARG(o.pipeline == NW_PIPE_PLAIN || o.pipeline == NW_PIPE_FUSED, "bad pipeline");
+    if (o.pipeline == NW_PIPE_PLAIN && o.precision != NW_PREC_FP64) {
+        set_error("the plain (reference-decomposed) pipeline is FP64 only");
+        return NW_ERR_UNSUPPORTED;
+    }
+    if (o.pipeline == NW_PIPE_FUSED && !fused_supported(m.n1, o.precision)) {
+        set_error("fused pipeline supports n1 in [2, 9]");
+        return NW_ERR_UNSUPPORTED;
+    }
+    int ndev = 0;
+    NW_TRY_HIP(hipGetDeviceCount(&ndev));
+    NW_CHECK_ARG(o.device >= 0 && o.device < ndev, "bad device ordinal");
+    device = o.device;
+    NW_TRY_HIP(hipSetDevice(device));
+    int st = upload_gll_tables(device);
+    if (st) return st;
+    prec = o.precision;
+    pipe = o.pipeline;
+    use_graph = o.use_graph != 0;
+    option = o.tile_option < 0 ? 0 : (o.tile_option ? 1 : 0);
+    n1 = m.n1;
+    np = m.np;
+    nelt = m.nelt;
+    npts = m.npts();
+    tfp = m.tfp();
+    nhalo = m.nhalo_points();
+    stride = npts + nhalo;
+    c0 = c0_;
+    media = m.media;
+    NW_TRY_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    NW_TRY_HIP(hipEventCreate(&ev0));
+    NW_TRY_HIP(hipEventCreate(&ev1));
+
+    const size_t rsz = (prec == kFP64) ? 8 : 4;
+    const size_t ksz = (prec == kFP32) ? 4 : 8;
+    if ((st = dalloc(reinterpret_cast<void**>(&d_dt), 8))) return st;
+    if ((st = dalloc(reinterpret_cast<void**>(&d_diag_part), sizeof(double) * 3 * diag_blocks()))) return st;
+    if ((st = dalloc(reinterpret_cast<void**>(&d_diag_out), sizeof(double) * 3))) return st;
+    if ((st = dalloc(reinterpret_cast<void**>(&d_stage), sizeof(double) * 6 * npts))) return st;
+    if ((st = upload_vec(&d_mat, m.mat))) return st;
+    if ((st = upload_vec(&d_geom, m.geom))) return st;
+
+    // halo stand-in metadata (dg_solver.cpp:131-170 with one neighbour: self)
+    if (nhalo > 0) {
+        const int nh = static_cast<int>(nhalo);
+        std::vector<int> off(nh), sizes(nh, nh);
+        for (int j = 0; j < nh; ++j) off[j] = j;
+        if ((st = upload_vec(&d_send_vol, m.send_vol))) return st;
+        if ((st = upload_vec(&d_off, off))) return st;
+        if ((st = upload_vec(&d_sizes, sizes))) return st;
+        if ((st = upload_vec(&d_gmat, m.gmat))) return st;
+        // element owning each ghost slot (the flux kernel looks media up per point)
+        std::vector<int> ge(nh);
+        const int n2 = n1 * n1;
+        for (int j = 0; j < nh; ++j) ge[j] = m.ghost_elem[j / n2];
+        if ((st = upload_vec(&d_ghost_elem, ge))) return st;
+        if ((st = dalloc(&d_sendbuf, rsz * 6 * nh))) return st;
+        if ((st = dalloc(&d_recvbuf, rsz * 6 * nh))) return st;
+    }
+
+    if (pipe == NW_PIPE_FUSED) {
+        if ((st = dalloc(&d_u[0], rsz * 6 * stride))) return st;
+        if ((st = dalloc(&d_u[1], rsz * 6 * stride))) return st;
+        if ((st = dalloc(&d_k, ksz * 6 * npts))) return st;
+        NW_TRY_HIP(hipMemsetAsync(d_u[0], 0, rsz * 6 * stride, stream));
+        NW_TRY_HIP(hipMemsetAsync(d_u[1], 0, rsz * 6 * stride, stream));
+        NW_TRY_HIP(hipMemsetAsync(d_k, 0, ksz * 6 * npts, stream));
+        if ((st = upload_vec(&d_affine, m.affine))) return st;
+        if ((st = upload_vec(&d_fcode, m.fcode))) return st;
+        kernels_per_step = 5 * (1 + (nhalo > 0 ? 2 : 0));
+    } else {
+        // reference residency (dg_solver.cpp:114-236): state, k, rhs, D, w3,
+        // metrics, face arrays, face scratch -- geometry generated on the device
+        if ((st = dalloc(reinterpret_cast<void**>(&d_state), 8 * 6 * stride))) return st;
+        if ((st = dalloc(reinterpret_cast<void**>(&d_kk), 8 * 6 * npts))) return st;
+        if ((st = dalloc(reinterpret_cast<void**>(&d_rhs), 8 * 6 * npts))) return st;
+        NW_TRY_HIP(hipMemsetAsync(d_state, 0, 8 * 6 * stride, stream));
+        NW_TRY_HIP(hipMemsetAsync(d_kk, 0, 8 * 6 * npts, stream));
+        std::vector<double> Dcol(n1 * n1), w3(np);
+        const GllTables& T = host_gll_tables();
+        for (int i = 0; i < n1; ++i)
+            for (int l = 0; l < n1; ++l) Dcol[i + l * n1] = T.D[n1][i * n1 + l];
+        for (int k = 0; k < n1; ++k)
+            for (int j = 0; j < n1; ++j)
+                for (int i = 0; i < n1; ++i) w3[i + n1 * (j + n1 * k)] = T.w[n1][i] * T.w[n1][j] * T.w[n1][k];
+        if ((st = upload_vec(&d_D, Dcol))) return st;
+        if ((st = upload_vec(&d_w3, w3))) return st;
+        for (int a = 0; a < 13; ++a)
+            if ((st = dalloc(reinterpret_cast<void**>(&d_met[a]), 8 * npts))) return st;
+        double* corners = nullptr;
+        int* fcode = nullptr;
+        int* fbc = nullptr;
+        if ((st = upload_vec(&corners, m.corners))) return st;
+        if ((st = upload_vec(&fcode, m.fcode))) return st;
+        if ((st = upload_vec(&fbc, m.fbc))) return st;
+        NW_TRY_HIP(launch_metrics(corners, n1, npts, d_met, stream));
+        if ((st = dalloc(reinterpret_cast<void**>(&d_vM), 4 * tfp))) return st;
+        if ((st = dalloc(reinterpret_cast<void**>(&d_vP), 4 * tfp))) return st;
+        if ((st = dalloc(reinterpret_cast<void**>(&d_bc), 4 * tfp))) return st;
+        for (int a = 0; a < 4; ++a)
+            if ((st = dalloc(reinterpret_cast<void**>(&d_face[a]), 8 * tfp))) return st;
+        const double* met10[10] = {d_met[3], d_met[4], d_met[5], d_met[6], d_met[7],
+                                   d_met[8], d_met[9], d_met[10], d_met[11], d_met[12]};
+        NW_TRY_HIP(launch_face_data(fcode, fbc, n1, tfp, npts, met10, d_vM, d_vP, d_bc, d_face[0], d_face[1],
+                                    d_face[2], d_face[3], stream));
+        if ((st = dalloc(reinterpret_cast<void**>(&d_fEN), 8 * 3 * tfp))) return st;
+        if ((st = dalloc(reinterpret_cast<void**>(&d_fHN), 8 * 3 * tfp))) return st;
+        if ((st = dalloc(reinterpret_cast<void**>(&d_flux), 8 * 6 * tfp))) return st;
+        if (media) {
+            if ((st = upload_vec(&d_eps, m.eps))) return st;
+            if ((st = upload_vec(&d_mu, m.mu_))) return st;
+        }
+        kernels_per_step = 5 * (7 + (nhalo > 0 ? 2 : 0));
+    }
+    NW_TRY_HIP(hipStreamSynchronize(stream));
+    initialized = true;
+    return NW_OK;
+}
+
+int Solver::upload(const double* host, size_t n) {
+    if (!initialized) { set_error("upload before create"); return NW_ERR_STATE; }
+    NW_CHECK_ARG(host != nullptr && n == size_t(6 * npts), "upload: size must be 6*npts");
+    NW_TRY_HIP(hipSetDevice(device));
+    if (pipe == NW_PIPE_PLAIN) {
+        NW_TRY_HIP(hipMemcpy2DAsync(d_state, stride * 8, host, npts * 8, npts * 8, 6, hipMemcpyHostToDevice, stream));
+    } else {
+        cur = 0;
+        if (prec == kFP64) {
+            NW_TRY_HIP(hipMemcpy2DAsync(d_u[0], stride * 8, host, npts * 8, npts * 8, 6, hipMemcpyHostToDevice, stream));
+        } else {
+            NW_TRY_HIP(hipMemcpyAsync(d_stage, host, 8 * 6 * npts, hipMemcpyHostToDevice, stream));
+            NW_TRY_HIP((launch_convert<float, double>(static_cast<float*>(d_u[0]), stride, d_stage, npts, npts, 6, stream)));
+        }
+    }
+    NW_TRY_HIP(hipStreamSynchronize(stream));
+    uploaded = true;
+    return NW_OK;
+}
+
+int Solver::download(double* host, size_t n) {
+    if (!initialized) { set_error("download before create"); return NW_ERR_STATE; }
+    NW_CHECK_ARG(host != nullptr && n == size_t(6 * npts), "download: size must be 6*npts");
+    NW_TRY_HIP(hipSetDevice(device));
+    if (pipe == NW_PIPE_PLAIN) {
+        NW_TRY_HIP(hipMemcpy2DAsync(host, npts * 8, d_state, stride * 8, npts * 8, 6, hipMemcpyDeviceToHost, stream));
+    } else if (prec == kFP64) {
+        NW_TRY_HIP(hipMemcpy2DAsync(host, npts * 8, d_u[cur], stride * 8, npts * 8, 6, hipMemcpyDeviceToHost, stream));
+    } else {
+        NW_TRY_HIP((launch_convert<double, float>(d_stage, npts, static_cast<const float*>(d_u[cur]), stride, npts, 6, stream)));
+        NW_TRY_HIP(hipMemcpyAsync(host, d_stage, 8 * 6 * npts, hipMemcpyDeviceToHost, stream));
+    }
+    NW_TRY_HIP(hipStreamSynchronize(stream));
+    return NW_OK;
+}
+
+int Solver::set_dt(double dt) {
+    if (dt == last_dt) return NW_OK;
+    NW_TRY_HIP(hipStreamSynchronize(stream));
+    NW_TRY_HIP(hipMemcpy(d_dt, &dt, sizeof(double), hipMemcpyHostToDevice));
+    last_dt = dt;
+    return NW_OK;
+}
+
+// One stage of the fused pipeline, reading d_u[parity] and writing d_u[1-parity].
+int Solver::enqueue_fused_stage(int s, int parity) {
+    void* uin = d_u[parity];
+    if (nhalo > 0) {
+        const int nh = static_cast<int>(nhalo);
+        if (prec == kFP64) {
+            NW_TRY_HIP(launch_pack<double>(static_cast<const double*>(uin), static_cast<double*>(d_sendbuf), d_send_vol,
+                                           d_off, d_sizes, nh, stride, stream));
+        } else {
+            NW_TRY_HIP(launch_pack<float>(static_cast<const float*>(uin), static_cast<float*>(d_sendbuf), d_send_vol,
+                                          d_off, d_sizes, nh, stride, stream));
+        }
+        const size_t rsz = (prec == kFP64) ? 8 : 4;
+        NW_TRY_HIP(hipMemcpyAsync(d_recvbuf, d_sendbuf, rsz * 6 * nhalo, hipMemcpyDeviceToDevice, stream));
+        if (prec == kFP64) {
+            NW_TRY_HIP(launch_unpack<double>(static_cast<const double*>(d_recvbuf), static_cast<double*>(uin), d_off,
+                                             d_sizes, nh, npts, stride, stream));
+        } else {
+            NW_TRY_HIP(launch_unpack<float>(static_cast<const float*>(d_recvbuf), static_cast<float*>(uin), d_off,
+                                            d_sizes, nh, npts, stride, stream));
+        }
+    }
+    FusedArgs a{};
+    a.uin = uin;
+    a.uout = d_u[1 - parity];
+    a.kbuf = d_k;
+    a.stride = stride;
+    a.kstride = npts;
+    a.node_base = 0;
+    a.ghost_base = npts;
+    a.nelt = nelt;
+    a.geom = d_geom;
+    a.affine = d_affine;
+    a.mat = d_mat;
+    a.gmat = d_gmat;
+    a.fcode = d_fcode;
+    a.dt = d_dt;
+    a.a = kRkA[s];
+    a.b = kRkB[s];
+    a.c0 = c0;
+    a.first_stage = (s == 0);
+    a.weighted = media ? 1 : 0;
+    NW_TRY_HIP(launch_fused(n1, prec, option, a, stream));
+    return NW_OK;
+}
+
+int Solver::enqueue_plain_stage(int s) {
+    double* st = d_state;
+    const long long S = stride;
+    if (nhalo > 0) {
+        const int nh = static_cast<int>(nhalo);
+        NW_TRY_HIP(launch_pack<double>(st, static_cast<double*>(d_sendbuf), d_send_vol, d_off, d_sizes, nh, S, stream));
+        NW_TRY_HIP(hipMemcpyAsync(d_recvbuf, d_sendbuf, 8 * 6 * nhalo, hipMemcpyDeviceToDevice, stream));
+    }
+    PlainVolumeArgs v{};
+    v.D = d_D; v.w3g = d_w3; v.jac = d_met[3];
+    v.rx = d_met[4]; v.sx = d_met[5]; v.tx = d_met[6]; v.ry = d_met[7]; v.sy = d_met[8]; v.ty = d_met[9];
+    v.rz = d_met[10]; v.sz = d_met[11]; v.tz = d_met[12];
+    v.n1 = n1; v.nelt = nelt;
+    // +curl H -> rhs_E ; -curl E -> rhs_H (dg_solver.cpp:393-413)
+    v.u1 = st + 3 * S; v.u2 = st + 4 * S; v.u3 = st + 5 * S;
+    v.w1 = d_rhs; v.w2 = d_rhs + npts; v.w3 = d_rhs + 2 * npts; v.sign = 1.0;
+    NW_TRY_HIP(launch_volume_curl(v, stream));
+    v.u1 = st; v.u2 = st + S; v.u3 = st + 2 * S;
+    v.w1 = d_rhs + 3 * npts; v.w2 = d_rhs + 4 * npts; v.w3 = d_rhs + 5 * npts; v.sign = -1.0;
+    NW_TRY_HIP(launch_volume_curl(v, stream));
+    if (nhalo > 0) {
+        NW_TRY_HIP(launch_unpack<double>(static_cast<const double*>(d_recvbuf), st, d_off, d_sizes,
+                                         static_cast<int>(nhalo), npts, S, stream));
+    }
+    NW_TRY_HIP(launch_restrict(st, d_fEN, d_fHN, d_vM, tfp, S, stream));
+    NW_TRY_HIP(launch_flux(st, d_fEN, d_fHN, d_vP, d_bc, d_face[0], d_face[1], d_face[2], d_flux, c0, tfp, S, d_vM,
+                           media ? d_eps : nullptr, media ? d_mu : nullptr, d_ghost_elem, np, npts, stream));
+    NW_TRY_HIP(launch_lift(d_flux, d_face[3], d_rhs, tfp, npts, n1, stream));
+    NW_TRY_HIP(launch_inv_mass(d_rhs, d_met[3], d_w3, np, npts, media ? d_eps : nullptr, media ? d_mu : nullptr, stream));
+    NW_TRY_HIP(launch_lsrk(st, d_kk, d_rhs, kRkA[s], kRkB[s], d_dt, 0.0, npts, S, stream));
+    return NW_OK;
+}
+
+int Solver::enqueue_step(int parity) {
+    int st;
+    for (int s = 0; s < 5; ++s) {
+        if (pipe == NW_PIPE_FUSED) st = enqueue_fused_stage(s, (parity + s) & 1);
+        else st = enqueue_plain_stage(s);
+        if (st) return st;
+    }
+    return NW_OK;
+}
+
+int Solver::build_graph(int parity) {
+    hipGraph_t g = nullptr;
+    NW_TRY_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    int st = enqueue_step(parity);
+    hipError_t e = hipStreamEndCapture(stream, &g);
+    if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) {
+        set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        return NW_ERR_HIP;
+    }
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        return NW_ERR_HIP;
+    }
+    graph[parity] = g;
+    gexec[parity] = x;
+    return NW_OK;
+}
+
+int Solver::launch_steps(double dt, int nsteps) {
+    if (!initialized) { set_error("step before create"); return NW_ERR_STATE; }
+    if (!uploaded) { set_error("step before uploadState"); return NW_ERR_STATE; }
+    NW_CHECK_ARG(nsteps >= 0, "nsteps must be >= 0");
+    NW_TRY_HIP(hipSetDevice(device));
+    int st = set_dt(dt);
+    if (st) return st;
+    for (int n = 0; n < nsteps; ++n) {
+        const int parity = (pipe == NW_PIPE_FUSED) ? cur : 0;
+        if (use_graph) {
+            if (!gexec[parity] && (st = build_graph(parity))) return st;
+            NW_TRY_HIP(hipGraphLaunch(gexec[parity], stream));
+        } else {
+            if ((st = enqueue_step(parity))) return st;
+        }
+        if (pipe == NW_PIPE_FUSED) cur ^= 1;  // five stages flip the ping-pong parity
+    }
+    return NW_OK;
+}
+
+int Solver::step(double dt, int nsteps) {
+    int st = launch_steps(dt, nsteps);
+    if (st) return st;
+    NW_TRY_HIP(hipStreamSynchronize(stream));
+    return NW_OK;
+}
+
+int Solver::time(double dt, int nsteps, double* ms) {
+    NW_TRY_HIP(hipSetDevice(device));
+    int st = set_dt(dt);
+    if (st) return st;
+    // build graphs outside the timed region
+    if (use_graph) {
+        for (int p = 0; p < 2; ++p)
+            if (!gexec[p] && (pipe == NW_PIPE_FUSED || p == 0) && (st = build_graph(p))) return st;
+    }
+    NW_TRY_HIP(hipEventRecord(ev0, stream));
+    if ((st = launch_steps(dt, nsteps))) return st;
+    NW_TRY_HIP(hipEventRecord(ev1, stream));
+    NW_TRY_HIP(hipEventSynchronize(ev1));
+    float f = 0.0f;
+    NW_TRY_HIP(hipEventElapsedTime(&f, ev0, ev1));
+    *ms = f;
+    return NW_OK;
+}
+
+int Solver::diag(nw_diag* out) {
+    if (!initialized) { set_error("diag before create"); return NW_ERR_STATE; }
+    NW_TRY_HIP(hipSetDevice(device));
+    const ElemMat* mat = media ? d_mat : nullptr;
+    if (pipe == NW_PIPE_PLAIN) {
+        NW_TRY_HIP(launch_diag<double>(d_state, stride, npts, n1, d_met[3], nullptr, mat, d_diag_part, d_diag_out, stream));
+    } else if (prec == kFP64) {
+        NW_TRY_HIP(launch_diag<double>(static_cast<const double*>(d_u[cur]), stride, npts, n1, nullptr, d_geom, mat,
+                                       d_diag_part, d_diag_out, stream));
+    } else {
+        NW_TRY_HIP(launch_diag<float>(static_cast<const float*>(d_u[cur]), stride, npts, n1, nullptr, d_geom, mat,
+                                      d_diag_part, d_diag_out, stream));
+    }
+    double h[3];
+    NW_TRY_HIP(hipMemcpyAsync(h, d_diag_out, sizeof(h), hipMemcpyDeviceToHost, stream));
+    NW_TRY_HIP(hipStreamSynchronize(stream));
+    out->energy = h[0];
+    out->max_e = h[1];
+    out->max_h = h[2];
+    return NW_OK;
+}
+
+}  // namespace nw

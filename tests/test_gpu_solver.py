@@ -1,0 +1,186 @@
+"""Solver parity on the GPU through the C ABI (DgSolver replacement) against the
+FP64 oracle on the same seeded inputs.
+
+Tolerances (relative L2 with the J*w3 weight of 3dboxpec_test.cpp:76-119):
+  * FP64, plain and fused pipelines: <= 1e-12;
+  * FP32 fields (k FP32 or FP64):    <= 2e-5 after the stated steps;
+and the reference's KATs on the device: energy non-increasing with upwind
+(nekwave_test.cpp:212-293), eigenmode errors (3dboxpec_test.cpp:136).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import analytic as an
+import nekwave_amd as nw
+from gpu_util import gaussian_state, pair, random_state, rel_l2, run_oracle, run_product
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    # name: (n1, nel, bounds, periodic, jitter, seed, media, halo, c0)
+    "pec_tm110_p4": (5, (4, 4, 4), (0, 1, 0, 1, 0, 1), (0, 0, 0), 0.0, 0, False, False, 1.0),
+    "periodic_jitter_p3": (4, (3, 3, 3), (0, 1, 0, 1, 0, 1), (1, 1, 1), 0.1, 2, False, False, 1.0),
+    "periodic_jitter_p7": (8, (2, 2, 2), (0, 1, 0, 1, 0, 1), (1, 1, 1), 0.1, 2, False, False, 1.0),
+    "media_pec_p5": (6, (3, 2, 2), (0, 1.5, 0, 1, 0, 1), (0, 0, 0), 0.0, 0, True, False, 1.0),
+    "halo_media_p4": (5, (4, 2, 2), (0, 1, 0, 1, 0, 1), (1, 0, 0), 0.0, 0, True, True, 1.0),
+    "central_jitter_p2": (3, (3, 2, 2), (0, 1, 0, 1, 0, 1), (0, 1, 0), 0.05, 8, False, False, 0.0),
+}
+
+
+def _build(name):
+    n1, nel, b, per, jit, seed, media, halo, c0 = CASES[name]
+    nelt = nel[0] * nel[1] * nel[2]
+    eps = mu = None
+    if media:
+        rng = np.random.default_rng(5)
+        eps = np.exp(0.3 * rng.standard_normal(nelt))
+        mu = np.exp(0.3 * rng.standard_normal(nelt))
+    m, o = pair(n1, nel, b, per, jit, seed, eps, mu, halo)
+    return m, o, c0
+
+
+@pytest.mark.parametrize("pipeline", ["plain", "fused"])
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_fp64_matches_oracle(name, pipeline):
+    m, o, c0 = _build(name)
+    st0 = random_state(o, 3)
+    dt = 0.5 * o.auto_dt()
+    for nsteps in (1, 7):
+        ref = run_oracle(o, st0, dt, nsteps, c0)
+        got = run_product(m, st0, dt, nsteps, c0, "fp64", pipeline)
+        err = rel_l2(o, got, ref)
+        print("%s %s steps=%d relL2=%.3e" % (name, pipeline, nsteps, err))
+        assert err <= 1e-12, err
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp32_k64"])
+@pytest.mark.parametrize("name", ["pec_tm110_p4", "periodic_jitter_p7", "halo_media_p4"])
+def test_fp32_matches_oracle(name, precision):
+    m, o, c0 = _build(name)
+    st0 = gaussian_state(o, width=0.2)
+    dt = 0.5 * o.auto_dt()
+    ref = run_oracle(o, st0, dt, 20, c0)
+    got = run_product(m, st0, dt, 20, c0, precision, "fused")
+    err = rel_l2(o, got, ref)
+    print("%s %s relL2=%.3e" % (name, precision, err))
+    assert err <= 2e-5, err
+
+
+def test_graph_and_stream_paths_are_bitwise_equal_and_deterministic():
+    m, o, c0 = _build("periodic_jitter_p3")
+    st0 = random_state(o, 1)
+    dt = o.auto_dt()
+    for prec in ("fp64", "fp32"):
+        a = run_product(m, st0, dt, 5, c0, prec, "fused", use_graph=True)
+        b = run_product(m, st0, dt, 5, c0, prec, "fused", use_graph=False)
+        c = run_product(m, st0, dt, 5, c0, prec, "fused", use_graph=True)
+        assert np.array_equal(a, b) and np.array_equal(a, c), prec
+    a = run_product(m, st0, dt, 3, c0, "fp64", "plain", use_graph=True)
+    b = run_product(m, st0, dt, 3, c0, "fp64", "plain", use_graph=False)
+    assert np.array_equal(a, b)
+
+
+def test_graph_replays_with_host_work_between():
+    """Replays of the captured step, interleaved with eager work and syncs, each
+    equal to the eager result (graph replay hazard check)."""
+    m, o, c0 = _build("halo_media_p4")
+    st0 = random_state(o, 4)
+    dt = o.auto_dt()
+    ref = run_product(m, st0, dt, 4, c0, "fp64", "fused", use_graph=False)
+    s = nw.DgSolver(precision="fp64", pipeline="fused", use_graph=True)
+    s.initialize(m, c0)
+    for rep in range(3):
+        s.uploadState(st0)
+        for _ in range(4):
+            s.step(dt)
+            s.diagnostics()
+        assert np.array_equal(s.downloadState(), ref), rep
+    s.finalize()
+
+
+def test_tile_options_agree():
+    m, o, c0 = _build("media_pec_p5")
+    st0 = random_state(o, 2)
+    dt = o.auto_dt()
+    for prec in ("fp64", "fp32"):
+        a = run_product(m, st0, dt, 3, c0, prec, "fused", tile_option=0)
+        b = run_product(m, st0, dt, 3, c0, prec, "fused", tile_option=1)
+        assert np.array_equal(a, b), prec
+
+
+@pytest.mark.parametrize("pipeline,precision", [("plain", "fp64"), ("fused", "fp64"), ("fused", "fp32")])
+def test_device_diagnostics_match_oracle(pipeline, precision):
+    m, o, c0 = _build("media_pec_p5")
+    st0 = random_state(o, 6)
+    s = nw.DgSolver(precision=precision, pipeline=pipeline)
+    s.initialize(m, c0)
+    s.uploadState(st0)
+    d = s.diagnostics()
+    tol = 1e-13 if precision == "fp64" else 1e-6
+    assert abs(d["energy"] - o.energy(st0)) <= tol * o.energy(st0)
+    assert abs(d["max_e"] - o.max_e(st0)) <= tol * o.max_e(st0)
+    assert abs(d["max_h"] - o.max_h(st0)) <= tol * o.max_h(st0)
+    s.finalize()
+
+
+@pytest.mark.parametrize("pipeline,precision", [("plain", "fp64"), ("fused", "fp64"), ("fused", "fp32"),
+                                                ("fused", "fp32_k64")])
+def test_maxwell_stability_kat_on_device(pipeline, precision):
+    """nekwave_test.cpp:212-293: n1=4, 2x2x2 on [-0.5,0.5]^3, upwind, Ez=exp(-50 r^2),
+    dt=0.005, 5 steps, energy never above E0 + 1e-12 and finite."""
+    m = nw.Mesh(4)
+    m.createBoxMesh(2, 2, 2, -0.5, 0.5, -0.5, 0.5, -0.5, 0.5)
+    x, y, z = m.coords()
+    st = np.zeros(6 * x.size)
+    st[2 * x.size:3 * x.size] = np.exp(-50.0 * (x * x + y * y + z * z))
+    s = nw.DgSolver(precision=precision, pipeline=pipeline)
+    s.initialize(m, 1.0)
+    s.uploadState(st)
+    e0 = s.diagnostics()["energy"]
+    assert e0 > 0
+    t = 0.0
+    for _ in range(5):
+        s.step(0.005, t)
+        t += 0.005
+        e = s.diagnostics()["energy"]
+        slack = 1e-12 if precision == "fp64" else 1e-6 * e0
+        assert e <= e0 + slack and math.isfinite(e)
+    s.finalize()
+
+
+@pytest.mark.parametrize("pipeline,precision", [("plain", "fp64"), ("fused", "fp64"), ("fused", "fp32")])
+def test_tm110_c1_on_device(pipeline, precision):
+    """PEC unit cube 4^3, p = 4, CFL 0.5, upwind, TM110, 200 steps."""
+    m, o = pair(5, (4, 4, 4))
+    dt = 0.5 * o.dxmin()
+    st0 = an.pack(an.tm_mode(o.x, o.y, 0.0))
+    got = run_product(m, st0, dt, 200, 1.0, precision, pipeline)
+    err = an.rel_l2(an.dV(o), got, an.pack(an.tm_mode(o.x, o.y, 200 * dt)))
+    assert err < 2e-3, err
+
+
+def test_case_lifecycle_boxpec_mode():
+    """3dboxpec_test.cpp restated: Case with a 3x3x3 [-1,1]^3 box, order 4,
+    central flux, auto dt, 50 steps, relative L2 <= 0.05."""
+    cfg = nw.Config(order=4, nelx=3, nely=3, nelz=3, c0=0.0, num_steps=50, output_frequency=10)
+    c = nw.Case(cfg)
+    c.setInitialConditionHook(lambda x, y, z: an.boxpec_mode(x, y, z, 0.0))
+    res = {}
+
+    def post(case):
+        x, y, z = case.mesh.coords()
+        ex = an.pack(an.boxpec_mode(x, y, z, case.current_time))
+        jw = case.mesh.getW3()
+        o = __import__("pyoracle").Mesh.box(4, (3, 3, 3), (-1, 1, -1, 1, -1, 1))
+        res["err"] = an.rel_l2(an.dV(o), case.state, ex)
+        assert jw.size == 64
+
+    c.setPostprocessingHook(post)
+    c.run()
+    assert c.current_step == 50
+    assert res["err"] <= 0.05, res
+    assert len(c.history) == 5
+    energies = [h[4] for h in c.history]
+    assert max(energies) - min(energies) < 1e-3 * energies[0]  # central flux: conserved to LSRK error
