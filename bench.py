@@ -1,0 +1,230 @@
+"""Benchmark of the DG-SEM Maxwell LSRK45 step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp32]
+
+Workload (the configuration the headline metric is quoted on): periodic
+[0,1]^3, 16^3 hexahedra with interior vertices jittered by +-10% h (seed 2),
+polynomial degree 7 (n1 = 8, 512 nodes/element, 2,097,152 nodes x 6 fields),
+Gaussian pulse Ez = exp(-|x-0.5|^2/0.1^2), upwind flux, LSRK45 at the
+reference's automatic dt.  A step is one LSRK45 step (5 fused stage launches)
+over the whole mesh with the state resident in HBM.
+
+Prints one JSON line: DOF-updates/s of the FP32 fused path (value), the
+roofline of the fused stage kernel (algorithmic bytes per launch / mean launch
+time from HIP events on the solver's stream), the FP64 fused and FP64
+reference-decomposed ("plain") rates, the FP32 relative L2 error against the
+FP64 path after 1000 steps, and the CPU baseline (the oracle's FP64 step on
+the host cores, bounded sample).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "DOF-updates/s + % HBM roofline, FP32/batched vs FP64 plain, at stated L2 error"
+HBM_PEAK_GBS = 8000.0
+N1, NEL, JITTER, SEED = 8, (16, 16, 16), 0.1, 2
+
+
+def algorithmic_bytes_per_step(npts, nelt, s_u, s_k):
+    """B_step = DOF (10 s_u + 9 s_k) + 5 nelt (24*8 + 2*8): per step, stage 1
+    reads u, writes u and k; stages 2-5 read and write u and k; each stage reads
+    the element's 8 FP64 corners and its eps, mu."""
+    return 6 * npts * (10 * s_u + 9 * s_k) + 5 * nelt * (24 * 8 + 2 * 8)
+
+
+def build_mesh(nw):
+    m = nw.Mesh(N1)
+    m.createBoxMesh(*NEL, 0, 1, 0, 1, 0, 1, True, True, True, jitter=JITTER, seed=SEED)
+    return m
+
+
+def initial_state(x, y, z):
+    r2 = (x - 0.5) ** 2 + (y - 0.5) ** 2 + (z - 0.5) ** 2
+    zero = np.zeros_like(x)
+    return np.concatenate([zero, zero, np.exp(-r2 / 0.01), zero, zero, zero])
+
+
+def time_path(nw, m, st0, dt, precision, pipeline, steps, warmup):
+    s = nw.DgSolver(precision=precision, pipeline=pipeline, use_graph=True)
+    s.initialize(m, 1.0)
+    s.uploadState(st0)
+    if warmup:
+        s.run(dt, warmup)
+    s.uploadState(st0)
+    nw.lib().nw_dev_sync()
+    t0 = time.perf_counter()
+    ms = s.time(dt, steps)  # HIP events on the solver's stream around exactly `steps` steps
+    nw.lib().nw_dev_sync()
+    wall = time.perf_counter() - t0
+    info = s.info()
+    s.finalize()
+    return ms, wall, info
+
+
+def rel_l2(a, b, w):
+    n = w.size
+    num = sum(np.sum((a[c * n:(c + 1) * n] - b[c * n:(c + 1) * n]) ** 2 * w) for c in range(6))
+    den = sum(np.sum(b[c * n:(c + 1) * n] ** 2 * w) for c in range(6))
+    return math.sqrt(num / den)
+
+
+def cpu_baseline(dt, budget_s=15.0):
+    """The oracle (FP64 CPU restatement of the reference step) on the host
+    cores, on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+
+    threads = min(16, os.cpu_count() or 1)
+    po.set_threads(threads)
+    o = po.Mesh.box(N1, NEL, (0, 1, 0, 1, 0, 1), (1, 1, 1), jitter=JITTER, seed=SEED)
+    s = po.Solver(o, 1.0)
+    s.upload(initial_state(o.x, o.y, o.zc))
+    t1 = s.run(dt, 1)  # one step sizes the sample
+    n = max(1, min(40, int(budget_s / max(t1, 1e-3))))
+    t = s.run(dt, n)
+    cpu_model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": 6 * o.npts * n / t, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+            "sample": "%d LSRK45 step(s) of the same 16^3 n1=8 jittered periodic mesh, FP64, %d OpenMP "
+                      "threads (%s), %.1f s" % (n, threads, cpu_model, t)}
+
+
+def load_traffic(name):
+    p = os.path.join(ROOT, "profiles", "traffic_%s.json" % name)
+    if os.path.exists(p):
+        return json.load(open(p))
+    return None
+
+
+def profile_run(nw, precision, steps):
+    import ctypes
+
+    m = build_mesh(nw)
+    x, y, z = m.coords()
+    st0 = initial_state(x, y, z)
+    dt = m.computeAutomaticDt(1.0, 0.4)
+    s = nw.DgSolver(precision=precision, pipeline="fused", use_graph=False)
+    s.initialize(m, 1.0)
+    s.uploadState(st0)
+    s.run(dt, steps)
+    s.finalize()
+    ms = ctypes.c_double()
+    nw._lib.check(nw.lib().nw_bench_copy(256 << 20, 3, ctypes.byref(ms)))
+    print("profile run: %d steps (%d fused launches), 4 copies of 256 MiB" % (steps, 5 * steps), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32_k64", "fp64"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--accuracy-steps", type=int, default=1000)
+    ap.add_argument("--profile-run", action="store_true",
+                    help="short run for rocprofv3 counter passes: the fused stage on the workload "
+                         "plus a 256 MiB device copy as the HBM byte-count calibration")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the multi-GPU halo path is design-only); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import nekwave_amd as nw
+
+    if args.profile_run:
+        profile_run(nw, args.precision, steps=min(args.steps, 4))
+        return
+
+    dev = nw.device_info(0)
+    m = build_mesh(nw)
+    info = m.info()
+    npts, nelt = info.npts, info.nelt
+    x, y, z = m.coords()
+    st0 = initial_state(x, y, z)
+    dt = m.computeAutomaticDt(1.0, 0.4)
+    dof = 6 * npts
+
+    ms, wall, sinfo = time_path(nw, m, st0, dt, args.precision, "fused", args.steps, args.warmup)
+    value = dof * args.steps / (ms * 1e-3)
+    s_u = 8 if args.precision == "fp64" else 4
+    s_k = 4 if args.precision == "fp32" else 8
+    b_step = algorithmic_bytes_per_step(npts, nelt, s_u, s_k)
+    launches = sinfo["kernels_per_step"] * args.steps
+    per_launch_ms = ms / launches
+    achieved = (b_step / 5) / (per_launch_ms * 1e-3) / 1e9
+    traffic = load_traffic("fused_stage_%s_c2" % args.precision)
+
+    # comparison paths (fewer steps: they are slower)
+    cmp_steps = max(10, args.steps // 5)
+    ms64, _, _ = time_path(nw, m, st0, dt, "fp64", "fused", cmp_steps, 5)
+    ms64p, _, _ = time_path(nw, m, st0, dt, "fp64", "plain", cmp_steps, 5)
+    v64 = dof * cmp_steps / (ms64 * 1e-3)
+    v64p = dof * cmp_steps / (ms64p * 1e-3)
+
+    # stated error: the measured precision vs the FP64 path after 1000 steps
+    acc = None
+    if args.accuracy_steps > 0 and args.precision != "fp64":
+        outs = {}
+        for prec in (args.precision, "fp64"):
+            s = nw.DgSolver(precision=prec, pipeline="fused")
+            s.initialize(m, 1.0)
+            s.uploadState(st0)
+            s.run(dt, args.accuracy_steps)
+            outs[prec] = s.downloadState()
+            s.finalize()
+        w = m.getJac() * np.tile(m.getW3(), nelt)  # dV = J w3 (3dboxpec_test.cpp:84)
+        acc = rel_l2(outs[args.precision], outs["fp64"], w)
+
+    cpu = None if args.no_cpu else cpu_baseline(dt)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if args.precision == "fp32" else ("fp32+fp64k" if args.precision == "fp32_k64" else "fp64"),
+        "data": "synthetic",
+        "config": {"workload": "periodic [0,1]^3 16^3 hexes, interior vertices jittered +-10% h (seed 2), "
+                               "N=7 (n1=8), Gaussian Ez pulse, upwind, LSRK45 auto dt",
+                   "nodes": npts, "dof": dof, "elements": nelt, "dt": dt, "pipeline": "fused",
+                   "precision": args.precision, "elements_per_workgroup": sinfo["epw"],
+                   "device": dev["arch"], "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "kernel": "k_fused_stage<8,*,%s>" % args.precision,
+                     "algorithmic_bytes_per_launch": b_step / 5, "mean_launch_ms": per_launch_ms},
+        "fp64_fused": {"value": v64, "ms_per_step": ms64 / cmp_steps},
+        "fp64_plain": {"value": v64p, "ms_per_step": ms64p / cmp_steps},
+        "speedup_vs_fp64_plain": value / v64p,
+        "rel_l2_vs_fp64": acc,
+        "rel_l2_steps": args.accuracy_steps,
+        "wall_s_timed": wall,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -94,6 +94,8 @@ int nw_mesh_get_info(const nw_mesh* m, nw_mesh_info* out);
 int nw_mesh_gll(const nw_mesh* m, double* z, double* w, double* D, double* w3);
 /* getCoordX/Y/Z (npts each), computed on the host. */
 int nw_mesh_coords(const nw_mesh* m, double* x, double* y, double* z);
+/* Jacobian determinant per node (getJac), npts values, computed on the host. */
+int nw_mesh_jacobian(const nw_mesh* m, double* jac);
 /* Element corners [nelt][8][3] (getElementCorners). */
 int nw_mesh_corners(const nw_mesh* m, double* corners);
 /* computeMinNodeDistance (mesh.cpp:883-930). */
@@ -141,11 +143,24 @@ typedef struct nw_diag {
 int nw_dg_diag(nw_dg* s, nw_diag* out);
 
 typedef struct nw_dg_info {
-    int n1, nelt, precision, pipeline, use_graph, epw;
+    int n1, nelt, precision, pipeline, use_graph, epw;  /* n1 = 0 for a batch of mixed n1 */
     int64_t npts, stride, device_bytes;
-    int kernels_per_step;
+    int kernels_per_step, instances, segments;
 } nw_dg_info;
 int nw_dg_get_info(const nw_dg* s, nw_dg_info* out);
+/* Extension (batched cavities): one solver advancing `count` independent meshes of
+ * any mix of n1 with a common dt.  Instances of equal n1 form one segment; each
+ * stage launches one fused kernel per segment, as concurrent graph branches.
+ * Instance i is addressed by its index in `meshes`. */
+int nw_batch_create(int count, const nw_mesh* const* meshes, double c0, const nw_opts* opts, nw_dg** out);
+int nw_dg_upload_instance(nw_dg* s, int instance, const double* host, size_t n);
+int nw_dg_download_instance(nw_dg* s, int instance, double* host, size_t n);
+int nw_dg_diag_instance(nw_dg* s, int instance, nw_diag* out);
+/* Autotuner: time each elements-per-workgroup option per segment and keep the
+ * fastest (runs by itself before the first step when tile_option = -1). */
+int nw_dg_tune(nw_dg* s);
+/* n1, element count and chosen elements per workgroup of segment `seg`. */
+int nw_dg_segment_info(const nw_dg* s, int seg, int* n1, int* nelt, int* epw);
 /* DgSolver::finalize (dg_solver.cpp:529-582) + destructor. */
 void nw_dg_destroy(nw_dg* s);
 

@@ -21,7 +21,7 @@ import numpy as np
 from . import _lib
 from ._lib import FACE_E, FACE_PEC, FACE_PERIODIC, FACE_PMC, NekWaveError, check, lib
 
-__all__ = ["Mesh", "DgSolver", "Case", "Config", "DeviceBuffer", "NekWaveError", "device_info",
+__all__ = ["Mesh", "DgSolver", "BatchSolver", "Case", "Config", "DeviceBuffer", "NekWaveError", "device_info",
            "FACE_E", "FACE_PERIODIC", "FACE_PEC", "FACE_PMC"]
 
 
@@ -157,6 +157,11 @@ class Mesh:
 
     def getCoordZ(self):
         return self.coords()[2]
+
+    def getJac(self):
+        j = np.zeros(self.getTotalPoints())
+        check(lib().nw_mesh_jacobian(self.handle, _dp(j)))
+        return j
 
     def getElementCorners(self):
         c = np.zeros(self.getNumElements() * 24)
@@ -301,6 +306,53 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+class BatchSolver(DgSolver):
+    """Many independent meshes (any mix of n1, common dt) in one solver: each
+    stage launches one fused kernel per n1 segment, as concurrent graph
+    branches.  Instances are addressed by their index in the list given to
+    initialize()."""
+
+    def __init__(self, precision="fp32", use_graph=True, device=0, tile_option=-1):
+        super().__init__(precision=precision, pipeline="fused", use_graph=use_graph, device=device,
+                         tile_option=tile_option)
+        self.sizes = []
+
+    def initialize(self, meshes, c0):
+        self.finalize()
+        arr = (C.c_void_p * len(meshes))(*[m.handle.value for m in meshes])
+        h = C.c_void_p()
+        check(lib().nw_batch_create(len(meshes), arr, float(c0), C.byref(self.opts), C.byref(h)))
+        self._h = h
+        self.sizes = [m.getTotalPoints() for m in meshes]
+        self.npts = self.sizes[0]
+        return True
+
+    def uploadState(self, instance, hostState):
+        a = np.ascontiguousarray(hostState, dtype=np.float64).reshape(-1)
+        check(lib().nw_dg_upload_instance(self._handle(), int(instance), _dp(a), a.size))
+
+    def downloadState(self, instance):
+        out = np.zeros(6 * self.sizes[instance])
+        check(lib().nw_dg_download_instance(self._handle(), int(instance), _dp(out), out.size))
+        return out
+
+    def diagnostics(self, instance=0):
+        o = _lib.Diag()
+        check(lib().nw_dg_diag_instance(self._handle(), int(instance), C.byref(o)))
+        return {"energy": o.energy, "max_e": o.max_e, "max_h": o.max_h}
+
+    def tune(self):
+        check(lib().nw_dg_tune(self._handle()))
+
+    def segments(self):
+        out = []
+        for s in range(self.info()["segments"]):
+            n1, nelt, epw = C.c_int(), C.c_int(), C.c_int()
+            check(lib().nw_dg_segment_info(self._handle(), s, C.byref(n1), C.byref(nelt), C.byref(epw)))
+            out.append({"n1": n1.value, "nelt": nelt.value, "epw": epw.value})
+        return out
 
 
 from .case import Case, Config  # noqa: E402

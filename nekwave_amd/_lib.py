@@ -40,7 +40,8 @@ class Diag(C.Structure):
 class DgInfo(C.Structure):
     _fields_ = [("n1", C.c_int), ("nelt", C.c_int), ("precision", C.c_int), ("pipeline", C.c_int),
                 ("use_graph", C.c_int), ("epw", C.c_int), ("npts", C.c_int64), ("stride", C.c_int64),
-                ("device_bytes", C.c_int64), ("kernels_per_step", C.c_int)]
+                ("device_bytes", C.c_int64), ("kernels_per_step", C.c_int), ("instances", C.c_int),
+                ("segments", C.c_int)]
 
 
 _lib = None
@@ -60,6 +61,7 @@ SIGNATURES = {
     "nw_mesh_gll": (i, [vp, dp, dp, dp, dp]),
     "nw_mesh_coords": (i, [vp, dp, dp, dp]),
     "nw_mesh_corners": (i, [vp, dp]),
+    "nw_mesh_jacobian": (i, [vp, dp]),
     "nw_mesh_min_node_distance": (i, [vp, dp]),
     "nw_mesh_auto_dt": (i, [vp, d, d, dp]),
     "nw_mesh_destroy": (None, [vp]),
@@ -74,6 +76,12 @@ SIGNATURES = {
     "nw_dg_diag": (i, [vp, C.POINTER(Diag)]),
     "nw_dg_get_info": (i, [vp, C.POINTER(DgInfo)]),
     "nw_dg_destroy": (None, [vp]),
+    "nw_batch_create": (i, [i, C.POINTER(vp), d, C.POINTER(Opts), C.POINTER(vp)]),
+    "nw_dg_upload_instance": (i, [vp, i, dp, sz]),
+    "nw_dg_download_instance": (i, [vp, i, dp, sz]),
+    "nw_dg_diag_instance": (i, [vp, i, C.POINTER(Diag)]),
+    "nw_dg_tune": (i, [vp]),
+    "nw_dg_segment_info": (i, [vp, i, ip, ip, ip]),
     "nw_launch_volume_curl": (i, [vp] * 18 + [d, i, i, vp]),
     "nw_launch_restrict_faces": (i, [vp, vp, vp, vp, i64, i64, vp]),
     "nw_launch_compute_flux": (i, [vp] * 9 + [d, i64, i64, vp, vp, vp, vp, i, i64, vp]),

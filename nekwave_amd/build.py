@@ -59,9 +59,25 @@ def build(verbose=True, jobs=None):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    build_cpp_kats()
     if verbose:
         print("built", LIB)
     return LIB
+
+
+KATS_SRC = os.path.join(ROOT, "tests", "cpp", "reference_kats.cpp")
+KATS_BIN = os.path.join(HERE, "nw_reference_kats")
+
+
+def build_cpp_kats(out=KATS_BIN):
+    """The reference's C++ test cases against the C++ host header, linked to the library."""
+    cxx = shutil.which("g++") or "g++"
+    cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), KATS_SRC,
+           "-L" + HERE, "-lnekwave_amd", "-Wl,-rpath,$ORIGIN", "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("C++ host interface build failed: %s\n%s" % (" ".join(cmd), r.stderr))
+    return out
 
 
 if __name__ == "__main__":

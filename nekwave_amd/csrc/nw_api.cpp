@@ -3,6 +3,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "nw_fused.hpp"
 #include "nw_plain.hpp"
@@ -138,6 +139,14 @@ NW_API int nw_mesh_coords(const nw_mesh* m, double* x, double* y, double* z) {
     NW_GUARD_END
 }
 
+NW_API int nw_mesh_jacobian(const nw_mesh* m, double* jac) {
+    NW_GUARD_BEGIN
+    NW_CHECK_ARG(m && jac, "null argument");
+    m->m.jacobian(jac);
+    return NW_OK;
+    NW_GUARD_END
+}
+
 NW_API int nw_mesh_corners(const nw_mesh* m, double* c) {
     NW_CHECK_ARG(m && c, "null argument");
     std::memcpy(c, m->m.corners.data(), sizeof(double) * m->m.corners.size());
@@ -181,7 +190,31 @@ NW_API int nw_dg_create(const nw_mesh* m, double c0, const nw_opts* opts, nw_dg*
     nw_opts_default(&o);
     if (opts) o = *opts;
     nw_dg* d = new nw_dg;
-    int st = d->s.init(m->m, c0, o);
+    const nw::Mesh* one = &m->m;
+    int st = d->s.init(&one, 1, c0, o);
+    if (st) {
+        delete d;
+        return st;
+    }
+    *out = d;
+    return NW_OK;
+    NW_GUARD_END
+}
+
+NW_API int nw_batch_create(int count, const nw_mesh* const* meshes, double c0, const nw_opts* opts, nw_dg** out) {
+    NW_GUARD_BEGIN
+    NW_CHECK_ARG(meshes && out && count >= 1, "bad argument");
+    *out = nullptr;
+    nw_opts o;
+    nw_opts_default(&o);
+    if (opts) o = *opts;
+    std::vector<const nw::Mesh*> ms(count);
+    for (int x = 0; x < count; ++x) {
+        NW_CHECK_ARG(meshes[x], "null mesh in batch");
+        ms[x] = &meshes[x]->m;
+    }
+    nw_dg* d = new nw_dg;
+    int st = d->s.init(ms.data(), count, c0, o);
     if (st) {
         delete d;
         return st;
@@ -193,11 +226,39 @@ NW_API int nw_dg_create(const nw_mesh* m, double c0, const nw_opts* opts, nw_dg*
 
 NW_API int nw_dg_upload(nw_dg* s, const double* host, size_t n) {
     NW_CHECK_ARG(s, "null solver");
-    return s->s.upload(host, n);
+    return s->s.upload(0, host, n);
 }
 NW_API int nw_dg_download(nw_dg* s, double* host, size_t n) {
     NW_CHECK_ARG(s, "null solver");
-    return s->s.download(host, n);
+    return s->s.download(0, host, n);
+}
+NW_API int nw_dg_upload_instance(nw_dg* s, int instance, const double* host, size_t n) {
+    NW_CHECK_ARG(s, "null solver");
+    return s->s.upload(instance, host, n);
+}
+NW_API int nw_dg_download_instance(nw_dg* s, int instance, double* host, size_t n) {
+    NW_CHECK_ARG(s, "null solver");
+    return s->s.download(instance, host, n);
+}
+NW_API int nw_dg_diag_instance(nw_dg* s, int instance, nw_diag* out) {
+    NW_CHECK_ARG(s && out, "null argument");
+    return s->s.diag(instance, out);
+}
+NW_API int nw_dg_tune(nw_dg* s) {
+    NW_CHECK_ARG(s, "null solver");
+    if (!s->s.uploaded) {
+        nw::set_error("tune needs an uploaded state");
+        return NW_ERR_STATE;
+    }
+    return s->s.tune();
+}
+NW_API int nw_dg_segment_info(const nw_dg* s, int seg, int* n1, int* nelt, int* epw) {
+    NW_CHECK_ARG(s && seg >= 0 && seg < (int)s->s.seg.size(), "bad segment");
+    const nw::Segment& g = s->s.seg[seg];
+    if (n1) *n1 = g.n1;
+    if (nelt) *nelt = g.nelt;
+    if (epw) *epw = nw::fused_epw(g.n1, g.option, s->s.prec);
+    return NW_OK;
 }
 NW_API int nw_dg_step(nw_dg* s, double dt, double time) {
     (void)time;
@@ -220,7 +281,7 @@ NW_API int nw_dg_sync(nw_dg* s) {
 }
 NW_API int nw_dg_diag(nw_dg* s, nw_diag* out) {
     NW_CHECK_ARG(s && out, "null argument");
-    return s->s.diag(out);
+    return s->s.diag(0, out);
 }
 NW_API int nw_dg_get_info(const nw_dg* s, nw_dg_info* o) {
     NW_CHECK_ARG(s && o, "null argument");
@@ -229,7 +290,9 @@ NW_API int nw_dg_get_info(const nw_dg* s, nw_dg_info* o) {
     o->precision = s->s.prec;
     o->pipeline = s->s.pipe;
     o->use_graph = s->s.use_graph ? 1 : 0;
-    o->epw = s->s.pipe == NW_PIPE_FUSED ? nw::fused_epw(s->s.n1, s->s.option, s->s.prec) : 1;
+    o->epw = s->s.pipe == NW_PIPE_FUSED ? nw::fused_epw(s->s.seg[0].n1, s->s.seg[0].option, s->s.prec) : 1;
+    o->instances = static_cast<int>(s->s.inst.size());
+    o->segments = static_cast<int>(s->s.seg.size());
     o->npts = s->s.npts;
     o->stride = s->s.stride;
     o->device_bytes = s->s.device_bytes;

@@ -19,12 +19,12 @@ struct FusedArgs {
     long long node_base;      // first node of this segment
     long long ghost_base;     // first ghost slot (node index) of this segment
     int nelt;                 // elements in this segment
+    int elem_base;            // global index of the segment's first element (into mat)
     int nchunk;               // blocks per XCD chunk (XCD-aware block remap)
     const ElemGeom* geom;     // per element, segment-local
-    const unsigned char* affine;
-    const ElemMat* mat;       // per element (always present; vacuum = 1)
+    const ElemMat* mat;       // per element, global index (always present; vacuum = 1)
     const ElemMat* gmat;      // per ghost face
-    const int* fcode;         // per element face (nelt*6)
+    const int4* fnb;          // per element face (nelt*6): neighbour addressing, see FaceNbr
     const double* dt;         // device scalar
     double a, b, c0;
     int first_stage;          // a == 0: k is not read
@@ -58,6 +58,12 @@ __host__ __device__ inline void face_ijk(int f, int n1, int p, int q, int& i, in
         default: i = p; j = q; k = n1 - 1; break;
     }
 }
+
+// Per element face: the plus-side node of face point (p, q) is
+//   x + p * y + q * z   (x < 0: mirror face, no plus side),
+// and w names the plus side's medium: >= 0 element index, < 0 ghost face -(w+1).
+int make_face_nbr(const int* fcode, int nelt, int n1, long long node_base, long long ghost_base, int elem_base,
+                  int4* out);
 
 // fcode encoding: >= 0 : nbr_elem * 64 + nbr_face * 8 + orient; -1 : mirror; <= -2 : ghost face -(c+2)
 __host__ __device__ inline int fcode_pack(int e, int f, int o) { return e * 64 + f * 8 + o; }

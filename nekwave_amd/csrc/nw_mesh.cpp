@@ -289,6 +289,17 @@ void Mesh::coords(double* X, double* Y, double* Z) const {
     }
 }
 
+void Mesh::jacobian(double* jac) const {
+    const GllTables& T = host_gll_tables();
+    const double* z = T.z[n1];
+    for (int e = 0; e < nelt; ++e) {
+        const double* c = &corners[size_t(e) * 24];
+        for (int k = 0; k < n1; ++k)
+            for (int j = 0; j < n1; ++j)
+                for (int i = 0; i < n1; ++i) jac[size_t(e) * np + i + n1 * (j + n1 * k)] = det_at(c, z[i], z[j], z[k]);
+    }
+}
+
 double Mesh::min_node_distance() const {
     const GllTables& T = host_gll_tables();
     const double* z = T.z[n1];

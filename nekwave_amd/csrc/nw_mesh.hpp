@@ -38,6 +38,7 @@ struct Mesh {
     long long tfp() const { return (long long)nelt * 6 * n1 * n1; }
     long long nhalo_points() const;
     void coords(double* x, double* y, double* z) const;
+    void jacobian(double* jac) const;
     double min_node_distance() const;
 };
 

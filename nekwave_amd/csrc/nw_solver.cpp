@@ -1,9 +1,12 @@
 // nw_solver.cpp -- the DgSolver replacement: device residency, stage
-// sequencing, graphs, diagnostics (dg_solver.cpp:40-582 is the reference).
+// sequencing, graphs, diagnostics (dg_solver.cpp:40-582 is the reference),
+// for one mesh or a batch of independent meshes of mixed n1.
 #include "nw_solver.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <mutex>
+#include <numeric>
 #include <vector>
 
 #include "nw_fused.hpp"
@@ -50,6 +53,12 @@ void Solver::release() {
     }
     for (void* p : allocs) (void)hipFree(p);
     allocs.clear();
+    for (hipEvent_t e : fork_ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : join_ev) (void)hipEventDestroy(e);
+    fork_ev.clear();
+    join_ev.clear();
+    for (hipStream_t s : aux) (void)hipStreamDestroy(s);
+    aux.clear();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     ev0 = ev1 = nullptr;
@@ -80,16 +89,28 @@ int Solver::upload_vec(T** dst, const std::vector<T>& v) {
     return NW_OK;
 }
 
-int Solver::init(const Mesh& m, double c0_, const nw_opts& o) {
+int Solver::init(const Mesh* const* meshes, int count, double c0_, const nw_opts& o) {
+    NW_CHECK_ARG(meshes && count >= 1, "need at least one mesh");
     NW_CHECK_ARG(o.precision >= 0 && o.precision < kNumPrec, "bad precision");
     NW_CHECK_ARG(o.pipeline == NW_PIPE_PLAIN || o.pipeline == NW_PIPE_FUSED, "bad pipeline");
     if (o.pipeline == NW_PIPE_PLAIN && o.precision != NW_PREC_FP64) {
         set_error("the plain (reference-decomposed) pipeline is FP64 only");
         return NW_ERR_UNSUPPORTED;
     }
-    if (o.pipeline == NW_PIPE_FUSED && !fused_supported(m.n1, o.precision)) {
-        set_error("fused pipeline supports n1 in [2, 9]");
+    if (o.pipeline == NW_PIPE_PLAIN && count != 1) {
+        set_error("the plain pipeline runs one mesh");
         return NW_ERR_UNSUPPORTED;
+    }
+    for (int x = 0; x < count; ++x) {
+        NW_CHECK_ARG(meshes[x] != nullptr, "null mesh in batch");
+        if (o.pipeline == NW_PIPE_FUSED && !fused_supported(meshes[x]->n1, o.precision)) {
+            set_error("fused pipeline supports n1 in [2, 9]");
+            return NW_ERR_UNSUPPORTED;
+        }
+        if (count > 1 && meshes[x]->nhalo_points() > 0) {
+            set_error("the halo stand-in runs a single mesh");
+            return NW_ERR_UNSUPPORTED;
+        }
     }
     int ndev = 0;
     NW_TRY_HIP(hipGetDeviceCount(&ndev));
@@ -101,31 +122,69 @@ int Solver::init(const Mesh& m, double c0_, const nw_opts& o) {
     prec = o.precision;
     pipe = o.pipeline;
     use_graph = o.use_graph != 0;
-    option = o.tile_option < 0 ? 0 : (o.tile_option ? 1 : 0);
-    n1 = m.n1;
-    np = m.np;
-    nelt = m.nelt;
-    npts = m.npts();
-    tfp = m.tfp();
-    nhalo = m.nhalo_points();
-    stride = npts + nhalo;
+    autotune = (o.tile_option < 0) && pipe == NW_PIPE_FUSED;
+    const int opt = o.tile_option < 0 ? 0 : (o.tile_option ? 1 : 0);
     c0 = c0_;
-    media = m.media;
     NW_TRY_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     NW_TRY_HIP(hipEventCreate(&ev0));
     NW_TRY_HIP(hipEventCreate(&ev1));
+
+    // layout: instances ordered by n1 so each n1 forms one contiguous segment
+    std::vector<int> order(count);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return meshes[a]->n1 < meshes[b]->n1; });
+    inst.assign(count, Instance{});
+    long long nodes = 0;
+    int elems = 0;
+    media = false;
+    for (int r : order) {
+        const Mesh& m = *meshes[r];
+        Instance& I = inst[r];
+        I.n1 = m.n1; I.np = m.np; I.nelt = m.nelt; I.elem_base = elems; I.node_base = nodes; I.npts = m.npts();
+        I.media = m.media;
+        media = media || m.media;
+        elems += m.nelt;
+        nodes += m.npts();
+        if (seg.empty() || seg.back().n1 != m.n1) {
+            Segment sg;
+            sg.n1 = m.n1;
+            sg.elem_base = I.elem_base;
+            sg.node_base = I.node_base;
+            sg.option = opt;
+            seg.push_back(sg);
+        }
+        seg.back().nelt += m.nelt;
+    }
+    nelt = elems;
+    npts = nodes;
+    n1 = count == 1 ? meshes[0]->n1 : 0;
+    np = count == 1 ? meshes[0]->np : 0;
+    tfp = count == 1 ? meshes[0]->tfp() : 0;
+    nhalo = count == 1 ? meshes[0]->nhalo_points() : 0;
+    stride = npts + nhalo;
 
     const size_t rsz = (prec == kFP64) ? 8 : 4;
     const size_t ksz = (prec == kFP32) ? 4 : 8;
     if ((st = dalloc(reinterpret_cast<void**>(&d_dt), 8))) return st;
     if ((st = dalloc(reinterpret_cast<void**>(&d_diag_part), sizeof(double) * 3 * diag_blocks()))) return st;
     if ((st = dalloc(reinterpret_cast<void**>(&d_diag_out), sizeof(double) * 3))) return st;
-    if ((st = dalloc(reinterpret_cast<void**>(&d_stage), sizeof(double) * 6 * npts))) return st;
-    if ((st = upload_vec(&d_mat, m.mat))) return st;
-    if ((st = upload_vec(&d_geom, m.geom))) return st;
+    long long max_inst = 0;
+    for (const Instance& I : inst) max_inst = std::max(max_inst, I.npts);
+    if ((st = dalloc(reinterpret_cast<void**>(&d_stage), sizeof(double) * 6 * max_inst))) return st;
+    {
+        std::vector<ElemMat> mat(nelt);
+        std::vector<ElemGeom> geom(nelt);
+        for (int r = 0; r < count; ++r) {
+            std::copy(meshes[r]->mat.begin(), meshes[r]->mat.end(), mat.begin() + inst[r].elem_base);
+            std::copy(meshes[r]->geom.begin(), meshes[r]->geom.end(), geom.begin() + inst[r].elem_base);
+        }
+        if ((st = upload_vec(&d_mat, mat))) return st;
+        if ((st = upload_vec(&d_geom, geom))) return st;
+    }
 
     // halo stand-in metadata (dg_solver.cpp:131-170 with one neighbour: self)
     if (nhalo > 0) {
+        const Mesh& m = *meshes[0];
         const int nh = static_cast<int>(nhalo);
         std::vector<int> off(nh), sizes(nh, nh);
         for (int j = 0; j < nh; ++j) off[j] = j;
@@ -133,105 +192,139 @@ int Solver::init(const Mesh& m, double c0_, const nw_opts& o) {
         if ((st = upload_vec(&d_off, off))) return st;
         if ((st = upload_vec(&d_sizes, sizes))) return st;
         if ((st = upload_vec(&d_gmat, m.gmat))) return st;
-        // element owning each ghost slot (the flux kernel looks media up per point)
+        // element owning each ghost slot (the plain flux kernel looks media up per point)
         std::vector<int> ge(nh);
-        const int n2 = n1 * n1;
+        const int n2 = m.n1 * m.n1;
         for (int j = 0; j < nh; ++j) ge[j] = m.ghost_elem[j / n2];
         if ((st = upload_vec(&d_ghost_elem, ge))) return st;
         if ((st = dalloc(&d_sendbuf, rsz * 6 * nh))) return st;
         if ((st = dalloc(&d_recvbuf, rsz * 6 * nh))) return st;
     }
 
-    if (pipe == NW_PIPE_FUSED) {
+    if (pipe == NW_PIPE_PLAIN) {
+        st = init_plain(*meshes[0]);
+        if (st) return st;
+    } else {
         if ((st = dalloc(&d_u[0], rsz * 6 * stride))) return st;
         if ((st = dalloc(&d_u[1], rsz * 6 * stride))) return st;
         if ((st = dalloc(&d_k, ksz * 6 * npts))) return st;
         NW_TRY_HIP(hipMemsetAsync(d_u[0], 0, rsz * 6 * stride, stream));
         NW_TRY_HIP(hipMemsetAsync(d_u[1], 0, rsz * 6 * stride, stream));
         NW_TRY_HIP(hipMemsetAsync(d_k, 0, ksz * 6 * npts, stream));
-        if ((st = upload_vec(&d_affine, m.affine))) return st;
-        if ((st = upload_vec(&d_fcode, m.fcode))) return st;
-        kernels_per_step = 5 * (1 + (nhalo > 0 ? 2 : 0));
-    } else {
-        // reference residency (dg_solver.cpp:114-236): state, k, rhs, D, w3,
-        // metrics, face arrays, face scratch -- geometry generated on the device
-        if ((st = dalloc(reinterpret_cast<void**>(&d_state), 8 * 6 * stride))) return st;
-        if ((st = dalloc(reinterpret_cast<void**>(&d_kk), 8 * 6 * npts))) return st;
-        if ((st = dalloc(reinterpret_cast<void**>(&d_rhs), 8 * 6 * npts))) return st;
-        NW_TRY_HIP(hipMemsetAsync(d_state, 0, 8 * 6 * stride, stream));
-        NW_TRY_HIP(hipMemsetAsync(d_kk, 0, 8 * 6 * npts, stream));
-        std::vector<double> Dcol(n1 * n1), w3(np);
-        const GllTables& T = host_gll_tables();
-        for (int i = 0; i < n1; ++i)
-            for (int l = 0; l < n1; ++l) Dcol[i + l * n1] = T.D[n1][i * n1 + l];
-        for (int k = 0; k < n1; ++k)
-            for (int j = 0; j < n1; ++j)
-                for (int i = 0; i < n1; ++i) w3[i + n1 * (j + n1 * k)] = T.w[n1][i] * T.w[n1][j] * T.w[n1][k];
-        if ((st = upload_vec(&d_D, Dcol))) return st;
-        if ((st = upload_vec(&d_w3, w3))) return st;
-        for (int a = 0; a < 13; ++a)
-            if ((st = dalloc(reinterpret_cast<void**>(&d_met[a]), 8 * npts))) return st;
-        double* corners = nullptr;
-        int* fcode = nullptr;
-        int* fbc = nullptr;
-        if ((st = upload_vec(&corners, m.corners))) return st;
-        if ((st = upload_vec(&fcode, m.fcode))) return st;
-        if ((st = upload_vec(&fbc, m.fbc))) return st;
-        NW_TRY_HIP(launch_metrics(corners, n1, npts, d_met, stream));
-        if ((st = dalloc(reinterpret_cast<void**>(&d_vM), 4 * tfp))) return st;
-        if ((st = dalloc(reinterpret_cast<void**>(&d_vP), 4 * tfp))) return st;
-        if ((st = dalloc(reinterpret_cast<void**>(&d_bc), 4 * tfp))) return st;
-        for (int a = 0; a < 4; ++a)
-            if ((st = dalloc(reinterpret_cast<void**>(&d_face[a]), 8 * tfp))) return st;
-        const double* met10[10] = {d_met[3], d_met[4], d_met[5], d_met[6], d_met[7],
-                                   d_met[8], d_met[9], d_met[10], d_met[11], d_met[12]};
-        NW_TRY_HIP(launch_face_data(fcode, fbc, n1, tfp, npts, met10, d_vM, d_vP, d_bc, d_face[0], d_face[1],
-                                    d_face[2], d_face[3], stream));
-        if ((st = dalloc(reinterpret_cast<void**>(&d_fEN), 8 * 3 * tfp))) return st;
-        if ((st = dalloc(reinterpret_cast<void**>(&d_fHN), 8 * 3 * tfp))) return st;
-        if ((st = dalloc(reinterpret_cast<void**>(&d_flux), 8 * 6 * tfp))) return st;
-        if (media) {
-            if ((st = upload_vec(&d_eps, m.eps))) return st;
-            if ((st = upload_vec(&d_mu, m.mu_))) return st;
+        std::vector<int4> fnb(size_t(nelt) * 6);
+        for (int r = 0; r < count; ++r) {
+            const Instance& I = inst[r];
+            if (make_face_nbr(meshes[r]->fcode.data(), I.nelt, I.n1, I.node_base, npts, I.elem_base,
+                              fnb.data() + size_t(I.elem_base) * 6)) {
+                set_error("node index beyond 2^31");
+                return NW_ERR_UNSUPPORTED;
+            }
         }
-        kernels_per_step = 5 * (7 + (nhalo > 0 ? 2 : 0));
+        if ((st = upload_vec(&d_fnb, fnb))) return st;
+        for (size_t x = 1; x < seg.size(); ++x) {
+            hipStream_t s2;
+            NW_TRY_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+            aux.push_back(s2);
+        }
+        for (size_t x = 0; x < seg.size(); ++x) {
+            hipEvent_t a, b;
+            NW_TRY_HIP(hipEventCreateWithFlags(&a, hipEventDisableTiming));
+            NW_TRY_HIP(hipEventCreateWithFlags(&b, hipEventDisableTiming));
+            fork_ev.push_back(a);
+            join_ev.push_back(b);
+        }
+        kernels_per_step = 5 * (static_cast<int>(seg.size()) + (nhalo > 0 ? 2 : 0));
     }
     NW_TRY_HIP(hipStreamSynchronize(stream));
     initialized = true;
     return NW_OK;
 }
 
-int Solver::upload(const double* host, size_t n) {
+// Reference residency (dg_solver.cpp:114-236): state, k, rhs, D, w3, metrics,
+// face arrays, face scratch -- with the geometry generated on the device.
+int Solver::init_plain(const Mesh& m) {
+    int st;
+    if ((st = dalloc(reinterpret_cast<void**>(&d_state), 8 * 6 * stride))) return st;
+    if ((st = dalloc(reinterpret_cast<void**>(&d_kk), 8 * 6 * npts))) return st;
+    if ((st = dalloc(reinterpret_cast<void**>(&d_rhs), 8 * 6 * npts))) return st;
+    NW_TRY_HIP(hipMemsetAsync(d_state, 0, 8 * 6 * stride, stream));
+    NW_TRY_HIP(hipMemsetAsync(d_kk, 0, 8 * 6 * npts, stream));
+    std::vector<double> Dcol(n1 * n1), w3(np);
+    const GllTables& T = host_gll_tables();
+    for (int i = 0; i < n1; ++i)
+        for (int l = 0; l < n1; ++l) Dcol[i + l * n1] = T.D[n1][i * n1 + l];
+    for (int k = 0; k < n1; ++k)
+        for (int j = 0; j < n1; ++j)
+            for (int i = 0; i < n1; ++i) w3[i + n1 * (j + n1 * k)] = T.w[n1][i] * T.w[n1][j] * T.w[n1][k];
+    if ((st = upload_vec(&d_D, Dcol))) return st;
+    if ((st = upload_vec(&d_w3, w3))) return st;
+    for (int a = 0; a < 13; ++a)
+        if ((st = dalloc(reinterpret_cast<void**>(&d_met[a]), 8 * npts))) return st;
+    double* corners = nullptr;
+    int* fcode = nullptr;
+    int* fbc = nullptr;
+    if ((st = upload_vec(&corners, m.corners))) return st;
+    if ((st = upload_vec(&fcode, m.fcode))) return st;
+    if ((st = upload_vec(&fbc, m.fbc))) return st;
+    NW_TRY_HIP(launch_metrics(corners, n1, npts, d_met, stream));
+    if ((st = dalloc(reinterpret_cast<void**>(&d_vM), 4 * tfp))) return st;
+    if ((st = dalloc(reinterpret_cast<void**>(&d_vP), 4 * tfp))) return st;
+    if ((st = dalloc(reinterpret_cast<void**>(&d_bc), 4 * tfp))) return st;
+    for (int a = 0; a < 4; ++a)
+        if ((st = dalloc(reinterpret_cast<void**>(&d_face[a]), 8 * tfp))) return st;
+    const double* met10[10] = {d_met[3], d_met[4], d_met[5], d_met[6],  d_met[7],
+                               d_met[8], d_met[9], d_met[10], d_met[11], d_met[12]};
+    NW_TRY_HIP(launch_face_data(fcode, fbc, n1, tfp, npts, met10, d_vM, d_vP, d_bc, d_face[0], d_face[1], d_face[2],
+                                d_face[3], stream));
+    if ((st = dalloc(reinterpret_cast<void**>(&d_fEN), 8 * 3 * tfp))) return st;
+    if ((st = dalloc(reinterpret_cast<void**>(&d_fHN), 8 * 3 * tfp))) return st;
+    if ((st = dalloc(reinterpret_cast<void**>(&d_flux), 8 * 6 * tfp))) return st;
+    if (media) {
+        if ((st = upload_vec(&d_eps, m.eps))) return st;
+        if ((st = upload_vec(&d_mu, m.mu_))) return st;
+    }
+    kernels_per_step = 5 * (7 + (nhalo > 0 ? 2 : 0));
+    return NW_OK;
+}
+
+int Solver::upload(int ix, const double* host, size_t n) {
     if (!initialized) { set_error("upload before create"); return NW_ERR_STATE; }
-    NW_CHECK_ARG(host != nullptr && n == size_t(6 * npts), "upload: size must be 6*npts");
+    NW_CHECK_ARG(ix >= 0 && ix < (int)inst.size(), "bad instance index");
+    const Instance& I = inst[ix];
+    NW_CHECK_ARG(host != nullptr && n == size_t(6 * I.npts), "upload: size must be 6*npts");
     NW_TRY_HIP(hipSetDevice(device));
+    const long long P = I.npts, S = stride;
     if (pipe == NW_PIPE_PLAIN) {
-        NW_TRY_HIP(hipMemcpy2DAsync(d_state, stride * 8, host, npts * 8, npts * 8, 6, hipMemcpyHostToDevice, stream));
+        NW_TRY_HIP(hipMemcpy2DAsync(d_state, S * 8, host, P * 8, P * 8, 6, hipMemcpyHostToDevice, stream));
+    } else if (prec == kFP64) {
+        double* dst = static_cast<double*>(d_u[cur]) + I.node_base;
+        NW_TRY_HIP(hipMemcpy2DAsync(dst, S * 8, host, P * 8, P * 8, 6, hipMemcpyHostToDevice, stream));
     } else {
-        cur = 0;
-        if (prec == kFP64) {
-            NW_TRY_HIP(hipMemcpy2DAsync(d_u[0], stride * 8, host, npts * 8, npts * 8, 6, hipMemcpyHostToDevice, stream));
-        } else {
-            NW_TRY_HIP(hipMemcpyAsync(d_stage, host, 8 * 6 * npts, hipMemcpyHostToDevice, stream));
-            NW_TRY_HIP((launch_convert<float, double>(static_cast<float*>(d_u[0]), stride, d_stage, npts, npts, 6, stream)));
-        }
+        NW_TRY_HIP(hipMemcpyAsync(d_stage, host, 8 * 6 * P, hipMemcpyHostToDevice, stream));
+        float* dst = static_cast<float*>(d_u[cur]) + I.node_base;
+        NW_TRY_HIP((launch_convert<float, double>(dst, S, d_stage, P, P, 6, stream)));
     }
     NW_TRY_HIP(hipStreamSynchronize(stream));
     uploaded = true;
     return NW_OK;
 }
 
-int Solver::download(double* host, size_t n) {
+int Solver::download(int ix, double* host, size_t n) {
     if (!initialized) { set_error("download before create"); return NW_ERR_STATE; }
-    NW_CHECK_ARG(host != nullptr && n == size_t(6 * npts), "download: size must be 6*npts");
+    NW_CHECK_ARG(ix >= 0 && ix < (int)inst.size(), "bad instance index");
+    const Instance& I = inst[ix];
+    NW_CHECK_ARG(host != nullptr && n == size_t(6 * I.npts), "download: size must be 6*npts");
     NW_TRY_HIP(hipSetDevice(device));
+    const long long P = I.npts, S = stride;
     if (pipe == NW_PIPE_PLAIN) {
-        NW_TRY_HIP(hipMemcpy2DAsync(host, npts * 8, d_state, stride * 8, npts * 8, 6, hipMemcpyDeviceToHost, stream));
+        NW_TRY_HIP(hipMemcpy2DAsync(host, P * 8, d_state, S * 8, P * 8, 6, hipMemcpyDeviceToHost, stream));
     } else if (prec == kFP64) {
-        NW_TRY_HIP(hipMemcpy2DAsync(host, npts * 8, d_u[cur], stride * 8, npts * 8, 6, hipMemcpyDeviceToHost, stream));
+        const double* src = static_cast<const double*>(d_u[cur]) + I.node_base;
+        NW_TRY_HIP(hipMemcpy2DAsync(host, P * 8, src, S * 8, P * 8, 6, hipMemcpyDeviceToHost, stream));
     } else {
-        NW_TRY_HIP((launch_convert<double, float>(d_stage, npts, static_cast<const float*>(d_u[cur]), stride, npts, 6, stream)));
-        NW_TRY_HIP(hipMemcpyAsync(host, d_stage, 8 * 6 * npts, hipMemcpyDeviceToHost, stream));
+        const float* src = static_cast<const float*>(d_u[cur]) + I.node_base;
+        NW_TRY_HIP((launch_convert<double, float>(d_stage, P, src, S, P, 6, stream)));
+        NW_TRY_HIP(hipMemcpyAsync(host, d_stage, 8 * 6 * P, hipMemcpyDeviceToHost, stream));
     }
     NW_TRY_HIP(hipStreamSynchronize(stream));
     return NW_OK;
@@ -245,7 +338,34 @@ int Solver::set_dt(double dt) {
     return NW_OK;
 }
 
+int Solver::launch_segment(const Segment& sg, int s, int parity, int option, hipStream_t st) {
+    FusedArgs a{};
+    a.uin = d_u[parity];
+    a.uout = d_u[1 - parity];
+    a.kbuf = d_k;
+    a.stride = stride;
+    a.kstride = npts;
+    a.node_base = sg.node_base;
+    a.ghost_base = npts;
+    a.nelt = sg.nelt;
+    a.elem_base = sg.elem_base;
+    a.geom = d_geom + sg.elem_base;
+    a.mat = d_mat;
+    a.gmat = d_gmat;
+    a.fnb = d_fnb + size_t(sg.elem_base) * 6;
+    a.dt = d_dt;
+    a.a = kRkA[s];
+    a.b = kRkB[s];
+    a.c0 = c0;
+    a.first_stage = (s == 0);
+    a.weighted = media ? 1 : 0;
+    NW_TRY_HIP(launch_fused(sg.n1, prec, option, a, st));
+    return NW_OK;
+}
+
 // One stage of the fused pipeline, reading d_u[parity] and writing d_u[1-parity].
+// Segments of different n1 are independent within a stage: with several, they
+// run as concurrent branches (forked from and joined back to the main stream).
 int Solver::enqueue_fused_stage(int s, int parity) {
     void* uin = d_u[parity];
     if (nhalo > 0) {
@@ -267,27 +387,18 @@ int Solver::enqueue_fused_stage(int s, int parity) {
                                             d_sizes, nh, npts, stride, stream));
         }
     }
-    FusedArgs a{};
-    a.uin = uin;
-    a.uout = d_u[1 - parity];
-    a.kbuf = d_k;
-    a.stride = stride;
-    a.kstride = npts;
-    a.node_base = 0;
-    a.ghost_base = npts;
-    a.nelt = nelt;
-    a.geom = d_geom;
-    a.affine = d_affine;
-    a.mat = d_mat;
-    a.gmat = d_gmat;
-    a.fcode = d_fcode;
-    a.dt = d_dt;
-    a.a = kRkA[s];
-    a.b = kRkB[s];
-    a.c0 = c0;
-    a.first_stage = (s == 0);
-    a.weighted = media ? 1 : 0;
-    NW_TRY_HIP(launch_fused(n1, prec, option, a, stream));
+    int st;
+    if (seg.size() == 1) return launch_segment(seg[0], s, parity, seg[0].option, stream);
+    NW_TRY_HIP(hipEventRecord(fork_ev[0], stream));
+    for (size_t x = 1; x < seg.size(); ++x) NW_TRY_HIP(hipStreamWaitEvent(aux[x - 1], fork_ev[0], 0));
+    for (size_t x = 0; x < seg.size(); ++x) {
+        hipStream_t sx = x == 0 ? stream : aux[x - 1];
+        if ((st = launch_segment(seg[x], s, parity, seg[x].option, sx))) return st;
+    }
+    for (size_t x = 1; x < seg.size(); ++x) {
+        NW_TRY_HIP(hipEventRecord(join_ev[x], aux[x - 1]));
+        NW_TRY_HIP(hipStreamWaitEvent(stream, join_ev[x], 0));
+    }
     return NW_OK;
 }
 
@@ -359,6 +470,37 @@ int Solver::build_graph(int parity) {
     return NW_OK;
 }
 
+// Autotuner: per n1 segment, time each elements-per-workgroup option on the
+// first-stage kernel (it writes u_out and k only, both rewritten by the next
+// real stage) and keep the fastest.  Runs at a step boundary, before graphs
+// are captured.
+int Solver::tune() {
+    if (pipe != NW_PIPE_FUSED) return NW_OK;
+    NW_TRY_HIP(hipSetDevice(device));
+    for (Segment& sg : seg) {
+        float best = 1e30f;
+        int bestopt = 0;
+        for (int opt = 0; opt < 2; ++opt) {
+            int st = launch_segment(sg, 0, cur, opt, stream);  // warm
+            if (st) return st;
+            NW_TRY_HIP(hipEventRecord(ev0, stream));
+            for (int r = 0; r < 3; ++r)
+                if ((st = launch_segment(sg, 0, cur, opt, stream))) return st;
+            NW_TRY_HIP(hipEventRecord(ev1, stream));
+            NW_TRY_HIP(hipEventSynchronize(ev1));
+            float ms = 0.0f;
+            NW_TRY_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+            if (ms < best) {
+                best = ms;
+                bestopt = opt;
+            }
+        }
+        sg.option = bestopt;
+    }
+    tuned = true;
+    return NW_OK;
+}
+
 int Solver::launch_steps(double dt, int nsteps) {
     if (!initialized) { set_error("step before create"); return NW_ERR_STATE; }
     if (!uploaded) { set_error("step before uploadState"); return NW_ERR_STATE; }
@@ -366,6 +508,7 @@ int Solver::launch_steps(double dt, int nsteps) {
     NW_TRY_HIP(hipSetDevice(device));
     int st = set_dt(dt);
     if (st) return st;
+    if (autotune && !tuned && nsteps > 0 && (st = tune())) return st;
     for (int n = 0; n < nsteps; ++n) {
         const int parity = (pipe == NW_PIPE_FUSED) ? cur : 0;
         if (use_graph) {
@@ -387,10 +530,12 @@ int Solver::step(double dt, int nsteps) {
 }
 
 int Solver::time(double dt, int nsteps, double* ms) {
+    if (!initialized || !uploaded) { set_error("time before create/upload"); return NW_ERR_STATE; }
     NW_TRY_HIP(hipSetDevice(device));
     int st = set_dt(dt);
     if (st) return st;
-    // build graphs outside the timed region
+    // tune and capture outside the timed region
+    if (autotune && !tuned && (st = tune())) return st;
     if (use_graph) {
         for (int p = 0; p < 2; ++p)
             if (!gexec[p] && (pipe == NW_PIPE_FUSED || p == 0) && (st = build_graph(p))) return st;
@@ -405,18 +550,21 @@ int Solver::time(double dt, int nsteps, double* ms) {
     return NW_OK;
 }
 
-int Solver::diag(nw_diag* out) {
+int Solver::diag(int ix, nw_diag* out) {
     if (!initialized) { set_error("diag before create"); return NW_ERR_STATE; }
+    NW_CHECK_ARG(ix >= 0 && ix < (int)inst.size(), "bad instance index");
+    const Instance& I = inst[ix];
     NW_TRY_HIP(hipSetDevice(device));
-    const ElemMat* mat = media ? d_mat : nullptr;
+    const ElemMat* mat = I.media ? d_mat + I.elem_base : nullptr;
+    const ElemGeom* geom = d_geom + I.elem_base;
     if (pipe == NW_PIPE_PLAIN) {
         NW_TRY_HIP(launch_diag<double>(d_state, stride, npts, n1, d_met[3], nullptr, mat, d_diag_part, d_diag_out, stream));
     } else if (prec == kFP64) {
-        NW_TRY_HIP(launch_diag<double>(static_cast<const double*>(d_u[cur]), stride, npts, n1, nullptr, d_geom, mat,
-                                       d_diag_part, d_diag_out, stream));
+        NW_TRY_HIP(launch_diag<double>(static_cast<const double*>(d_u[cur]) + I.node_base, stride, I.npts, I.n1,
+                                       nullptr, geom, mat, d_diag_part, d_diag_out, stream));
     } else {
-        NW_TRY_HIP(launch_diag<float>(static_cast<const float*>(d_u[cur]), stride, npts, n1, nullptr, d_geom, mat,
-                                      d_diag_part, d_diag_out, stream));
+        NW_TRY_HIP(launch_diag<float>(static_cast<const float*>(d_u[cur]) + I.node_base, stride, I.npts, I.n1,
+                                      nullptr, geom, mat, d_diag_part, d_diag_out, stream));
     }
     double h[3];
     NW_TRY_HIP(hipMemcpyAsync(h, d_diag_out, sizeof(h), hipMemcpyDeviceToHost, stream));

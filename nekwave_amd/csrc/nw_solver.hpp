@@ -1,4 +1,5 @@
-// nw_solver.hpp -- DgSolver replacement (device residency and stage sequencing).
+// nw_solver.hpp -- DgSolver replacement (device residency and stage sequencing)
+// for one mesh or a batch of independent meshes.
 #pragma once
 
 #include <vector>
@@ -7,16 +8,33 @@
 
 namespace nw {
 
+// One independent mesh inside the solver's buffers.
+struct Instance {
+    int n1 = 0, np = 0, nelt = 0, elem_base = 0;
+    long long node_base = 0, npts = 0;
+    bool media = false;
+};
+
+// A run of elements sharing n1: one fused launch per stage.
+struct Segment {
+    int n1 = 0, nelt = 0, elem_base = 0, option = 0;
+    long long node_base = 0;
+};
+
 struct Solver {
     int device = -1;
     hipStream_t stream = nullptr;
+    std::vector<hipStream_t> aux;  // one per extra segment: concurrent branches of a stage
+    std::vector<hipEvent_t> fork_ev, join_ev;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int prec = 0, pipe = 1, option = 0;
-    bool use_graph = true, initialized = false, uploaded = false, media = false;
+    int prec = 0, pipe = 1;
+    bool use_graph = true, initialized = false, uploaded = false, media = false, autotune = false, tuned = false;
     int n1 = 0, np = 0, nelt = 0, kernels_per_step = 0;
     long long npts = 0, tfp = 0, nhalo = 0, stride = 0, device_bytes = 0;
     double c0 = 1.0, last_dt = -1.0;
     int cur = 0;
+    std::vector<Instance> inst;  // in caller order
+    std::vector<Segment> seg;
     std::vector<void*> allocs;
 
     // shared
@@ -26,16 +44,15 @@ struct Solver {
     double* d_stage = nullptr;
     ElemMat* d_mat = nullptr;
     ElemGeom* d_geom = nullptr;
-    // halo stand-in
+    // halo stand-in (single mesh)
     int *d_send_vol = nullptr, *d_off = nullptr, *d_sizes = nullptr, *d_ghost_elem = nullptr;
     ElemMat* d_gmat = nullptr;
     void *d_sendbuf = nullptr, *d_recvbuf = nullptr;
     // fused pipeline
     void* d_u[2] = {nullptr, nullptr};
     void* d_k = nullptr;
-    unsigned char* d_affine = nullptr;
-    int* d_fcode = nullptr;
-    // plain pipeline (reference residency)
+    int4* d_fnb = nullptr;
+    // plain pipeline (reference residency, single mesh)
     double *d_state = nullptr, *d_kk = nullptr, *d_rhs = nullptr, *d_D = nullptr, *d_w3 = nullptr;
     double* d_met[13] = {};  // x y z jac rx sx tx ry sy ty rz sz tz
     int *d_vM = nullptr, *d_vP = nullptr, *d_bc = nullptr;
@@ -47,19 +64,22 @@ struct Solver {
 
     ~Solver();
     void release();
-    int init(const Mesh& m, double c0, const nw_opts& o);
-    int upload(const double* host, size_t n);
-    int download(double* host, size_t n);
+    int init(const Mesh* const* meshes, int count, double c0, const nw_opts& o);
+    int upload(int instance, const double* host, size_t n);
+    int download(int instance, double* host, size_t n);
     int step(double dt, int nsteps);
     int launch_steps(double dt, int nsteps);
     int time(double dt, int nsteps, double* ms);
-    int diag(nw_diag* out);
+    int diag(int instance, nw_diag* out);
+    int tune();
 
    private:
     int dalloc(void** p, size_t bytes);
     template <typename T>
     int upload_vec(T** dst, const std::vector<T>& v);
+    int init_plain(const Mesh& m);
     int set_dt(double dt);
+    int launch_segment(const Segment& sg, int s, int parity, int option, hipStream_t st);
     int enqueue_fused_stage(int s, int parity);
     int enqueue_plain_stage(int s);
     int enqueue_step(int parity);

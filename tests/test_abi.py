@@ -100,3 +100,12 @@ def test_solver_calls_before_initialize_raise():
     with pytest.raises(nw.NekWaveError) as e:
         s.step(0.1)
     assert e.value.status == _lib.NW_ERR_STATE
+
+
+def test_cpp_host_interface_compiles_and_links(tmp_path):
+    """include/nekwave_amd.hpp (the reference's Mesh/DgSolver/Case signatures)
+    builds against the library with the host compiler."""
+    from nekwave_amd import build as b
+
+    out = b.build_cpp_kats(str(tmp_path / "kats"))
+    assert os.path.exists(out)

@@ -1,0 +1,91 @@
+"""Batched cavities (many independent meshes of mixed n1 in one solver): each
+instance must advance exactly as it would alone -- bitwise equal to its own
+single-mesh solver run, and within the precision's bound of the FP64 oracle."""
+import numpy as np
+import pytest
+
+import analytic as an
+import nekwave_amd as nw
+import pyoracle as po
+from gpu_util import rel_l2, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+# (n1, k, eps, (m, n), phase)
+INSTANCES = [(4, 2, 2.0, (1, 1), 0.3), (6, 3, 1.0, (2, 1), 1.1), (4, 3, 3.5, (1, 3), 2.0),
+             (9, 2, 1.5, (3, 2), 0.0), (6, 2, 2.5, (2, 2), 4.0)]
+
+
+def _instance(n1, k, eps):
+    m = nw.Mesh(n1)
+    m.createBoxMesh(k, k, k, 0, 1, 0, 1, 0, 1)
+    m.setMaterials(np.full(k ** 3, eps), np.ones(k ** 3))
+    o = po.Mesh.box(n1, (k, k, k), (0, 1, 0, 1, 0, 1))
+    o.set_materials(np.full(k ** 3, eps), np.ones(k ** 3))
+    return m, o
+
+
+def _ic(o, eps, mn, phase):
+    return an.pack(an.tm_mode(o.x, o.y, 0.0, m=mn[0], n=mn[1], eps=eps, phase=phase))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_equals_single_runs_and_oracle(precision):
+    meshes, oracles, ics = [], [], []
+    for n1, k, eps, mn, ph in INSTANCES:
+        m, o = _instance(n1, k, eps)
+        meshes.append(m)
+        oracles.append(o)
+        ics.append(_ic(o, eps, mn, ph))
+    dt = min(o.auto_dt() for o in oracles)
+    b = nw.BatchSolver(precision=precision)
+    b.initialize(meshes, 1.0)
+    for x, st in enumerate(ics):
+        b.uploadState(x, st)
+    b.run(dt, 6)
+    info = b.info()
+    assert info["instances"] == len(INSTANCES) and info["segments"] == 3
+    segs = b.segments()
+    assert [s["n1"] for s in segs] == [4, 6, 9]
+    for x, (o, st) in enumerate(zip(oracles, ics)):
+        got = b.downloadState(x)
+        s = nw.DgSolver(precision=precision, pipeline="fused", tile_option=0)
+        s.initialize(meshes[x], 1.0)
+        s.uploadState(st)
+        s.run(dt, 6)
+        alone = s.downloadState()
+        s.finalize()
+        ref = run_oracle(o, st, dt, 6)
+        err = rel_l2(o, got, ref)
+        tol = 1e-12 if precision == "fp64" else 2e-5
+        assert err <= tol, (x, err)
+        # same element arithmetic as the single-mesh run (tile shape may differ
+        # after tuning, which does not change per-element arithmetic)
+        assert np.array_equal(got, alone), x
+        d = b.diagnostics(x)
+        assert abs(d["energy"] - o.energy(got)) <= 1e-6 * o.energy(got)
+    b.finalize()
+
+
+def test_batch_eigenmodes_match_closed_form():
+    """Each cavity carries its own TM_mn0 mode with its own eps: after a common
+    time every instance matches its closed form (omega = pi sqrt(m^2+n^2)/sqrt(eps))."""
+    meshes, oracles, ics = [], [], []
+    for n1, k, eps, mn, ph in INSTANCES:
+        m, o = _instance(n1, k, eps)
+        meshes.append(m)
+        oracles.append(o)
+        ics.append(_ic(o, eps, mn, ph))
+    dt = 0.5 * min(o.dxmin() for o in oracles)
+    nst = 40
+    b = nw.BatchSolver(precision="fp32")
+    b.initialize(meshes, 1.0)
+    for x, st in enumerate(ics):
+        b.uploadState(x, st)
+    b.run(dt, nst)
+    for x, (n1, k, eps, mn, ph) in enumerate(INSTANCES):
+        o = oracles[x]
+        ex = an.pack(an.tm_mode(o.x, o.y, nst * dt, m=mn[0], n=mn[1], eps=eps, phase=ph))
+        err = an.rel_l2(an.dV(o), b.downloadState(x), ex)
+        assert err < 0.02 * (mn[0] + mn[1]) / min(k, n1 - 1), (x, err)
+    b.finalize()

@@ -8,6 +8,7 @@ and the reference's KATs on the device: energy non-increasing with upwind
 (nekwave_test.cpp:212-293), eigenmode errors (3dboxpec_test.cpp:136).
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -184,3 +185,17 @@ def test_case_lifecycle_boxpec_mode():
     assert len(c.history) == 5
     energies = [h[4] for h in c.history]
     assert max(energies) - min(energies) < 1e-3 * energies[0]  # central flux: conserved to LSRK error
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_cpp_reference_kats_on_device(prec):
+    """tests/cpp/reference_kats.cpp through the C++ host header: the reference's
+    MaxwellStabilityTest and 3dboxpec Case lifecycle."""
+    import subprocess
+
+    from nekwave_amd import build as b
+
+    assert os.path.exists(b.KATS_BIN), "built by __graft_entry__.build()"
+    r = subprocess.run([b.KATS_BIN, prec], capture_output=True, text=True, timeout=100)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0 and "ALL PASSED" in r.stdout
