@@ -1,20 +1,21 @@
 """Benchmark of the DG-SEM Maxwell LSRK45 step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp32]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp32] [--configs c3,c4,c5]
 
-Workload (the configuration the headline metric is quoted on): periodic
-[0,1]^3, 16^3 hexahedra with interior vertices jittered by +-10% h (seed 2),
-polynomial degree 7 (n1 = 8, 512 nodes/element, 2,097,152 nodes x 6 fields),
-Gaussian pulse Ez = exp(-|x-0.5|^2/0.1^2), upwind flux, LSRK45 at the
-reference's automatic dt.  A step is one LSRK45 step (5 fused stage launches)
-over the whole mesh with the state resident in HBM.
+Headline workload (the configuration the metric is quoted on, "c2" in
+nekwave_amd/workloads.py): periodic [0,1]^3, 16^3 hexahedra with interior
+vertices jittered by +-10% h (seed 2), polynomial degree 7 (n1 = 8, 512
+nodes/element, 2,097,152 nodes x 6 fields), Gaussian pulse, upwind flux,
+LSRK45 at the reference's automatic dt.  A step is one LSRK45 step (5 fused
+stage launches) over the whole mesh with the state resident in HBM.
 
 Prints one JSON line: DOF-updates/s of the FP32 fused path (value), the
 roofline of the fused stage kernel (algorithmic bytes per launch / mean launch
 time from HIP events on the solver's stream), the FP64 fused and FP64
 reference-decomposed ("plain") rates, the FP32 relative L2 error against the
-FP64 path after 1000 steps, and the CPU baseline (the oracle's FP64 step on
-the host cores, bounded sample).
+FP64 path after 1000 steps, the other workloads (c3 PEC 32^3 with media, c4
+256 batched cavities, c5 64^3 share with the halo stand-in) and the CPU
+baseline (the oracle's FP64 step on the host cores, bounded sample).
 """
 import argparse
 import json
@@ -30,7 +31,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "DOF-updates/s + % HBM roofline, FP32/batched vs FP64 plain, at stated L2 error"
 HBM_PEAK_GBS = 8000.0
-N1, NEL, JITTER, SEED = 8, (16, 16, 16), 0.1, 2
+SIZES = {"fp64": (8, 8), "fp32": (4, 4), "fp32_k64": (4, 8)}
 
 
 def algorithmic_bytes_per_step(npts, nelt, s_u, s_k):
@@ -40,33 +41,41 @@ def algorithmic_bytes_per_step(npts, nelt, s_u, s_k):
     return 6 * npts * (10 * s_u + 9 * s_k) + 5 * nelt * (24 * 8 + 2 * 8)
 
 
-def build_mesh(nw):
-    m = nw.Mesh(N1)
-    m.createBoxMesh(*NEL, 0, 1, 0, 1, 0, 1, True, True, True, jitter=JITTER, seed=SEED)
-    return m
+def make_solver(nw, wl, precision, pipeline="fused"):
+    if len(wl.meshes) == 1:
+        s = nw.DgSolver(precision=precision, pipeline=pipeline, use_graph=True)
+        s.initialize(wl.meshes[0], wl.c0)
+        s.uploadState(wl.states[0])
+    else:
+        s = nw.BatchSolver(precision=precision, use_graph=True)
+        s.initialize(wl.meshes, wl.c0)
+        for x, st in enumerate(wl.states):
+            s.uploadState(x, st)
+    return s
 
 
-def initial_state(x, y, z):
-    r2 = (x - 0.5) ** 2 + (y - 0.5) ** 2 + (z - 0.5) ** 2
-    zero = np.zeros_like(x)
-    return np.concatenate([zero, zero, np.exp(-r2 / 0.01), zero, zero, zero])
+def reload(s, wl):
+    if len(wl.meshes) == 1:
+        s.uploadState(wl.states[0])
+    else:
+        for x, st in enumerate(wl.states):
+            s.uploadState(x, st)
 
 
-def time_path(nw, m, st0, dt, precision, pipeline, steps, warmup):
-    s = nw.DgSolver(precision=precision, pipeline=pipeline, use_graph=True)
-    s.initialize(m, 1.0)
-    s.uploadState(st0)
+def time_path(nw, wl, precision, pipeline, steps, warmup):
+    s = make_solver(nw, wl, precision, pipeline)
     if warmup:
-        s.run(dt, warmup)
-    s.uploadState(st0)
+        s.run(wl.dt, warmup)
+    reload(s, wl)
     nw.lib().nw_dev_sync()
     t0 = time.perf_counter()
-    ms = s.time(dt, steps)  # HIP events on the solver's stream around exactly `steps` steps
+    ms = s.time(wl.dt, steps)  # HIP events on the solver's stream around exactly `steps` steps
     nw.lib().nw_dev_sync()
     wall = time.perf_counter() - t0
     info = s.info()
+    segs = s.segments() if hasattr(s, "segments") else None
     s.finalize()
-    return ms, wall, info
+    return ms, wall, info, segs
 
 
 def rel_l2(a, b, w):
@@ -78,15 +87,18 @@ def rel_l2(a, b, w):
 
 def cpu_baseline(dt, budget_s=15.0):
     """The oracle (FP64 CPU restatement of the reference step) on the host
-    cores, on a bounded sample of the same workload."""
+    cores, on a bounded sample of the headline workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
 
     threads = min(16, os.cpu_count() or 1)
     po.set_threads(threads)
-    o = po.Mesh.box(N1, NEL, (0, 1, 0, 1, 0, 1), (1, 1, 1), jitter=JITTER, seed=SEED)
+    o = po.Mesh.box(8, (16, 16, 16), (0, 1, 0, 1, 0, 1), (1, 1, 1), jitter=0.1, seed=2)
+    x, y, z = o.x, o.y, o.zc
+    zero = np.zeros_like(x)
+    st = np.concatenate([zero, zero, np.exp(-((x - .5) ** 2 + (y - .5) ** 2 + (z - .5) ** 2) / 0.01), zero, zero, zero])
     s = po.Solver(o, 1.0)
-    s.upload(initial_state(o.x, o.y, o.zc))
+    s.upload(st)
     t1 = s.run(dt, 1)  # one step sizes the sample
     n = max(1, min(40, int(budget_s / max(t1, 1e-3))))
     t = s.run(dt, n)
@@ -110,21 +122,36 @@ def load_traffic(name):
     return None
 
 
-def profile_run(nw, precision, steps):
+def profile_run(nw, W, precision, steps):
     import ctypes
 
-    m = build_mesh(nw)
-    x, y, z = m.coords()
-    st0 = initial_state(x, y, z)
-    dt = m.computeAutomaticDt(1.0, 0.4)
-    s = nw.DgSolver(precision=precision, pipeline="fused", use_graph=False)
-    s.initialize(m, 1.0)
-    s.uploadState(st0)
-    s.run(dt, steps)
+    wl = W.c2()
+    s = nw.DgSolver(precision=precision, pipeline="fused", use_graph=False, tile_option=0)
+    s.initialize(wl.meshes[0], 1.0)
+    s.uploadState(wl.states[0])
+    s.run(wl.dt, steps)
     s.finalize()
     ms = ctypes.c_double()
     nw._lib.check(nw.lib().nw_bench_copy(256 << 20, 3, ctypes.byref(ms)))
     print("profile run: %d steps (%d fused launches), 4 copies of 256 MiB" % (steps, 5 * steps), flush=True)
+
+
+def other_config(nw, W, name, steps):
+    wl = W.BUILDERS[name]()
+    out = {"nodes": wl.npts, "elements": wl.nelt, "instances": len(wl.meshes), "description": wl.info["description"]}
+    precs = ["fp32", "fp64"] if name == "c5" else ["fp32"]
+    for prec in precs:
+        ms, _, info, segs = time_path(nw, wl, prec, "fused", steps, 2)
+        s_u, s_k = SIZES[prec]
+        b = algorithmic_bytes_per_step(wl.npts, wl.nelt, s_u, s_k)
+        out[prec] = {"dof_updates_per_s": 6 * wl.npts * steps / (ms * 1e-3), "ms_per_step": ms / steps,
+                     "algorithmic_GBps": b / (ms / steps * 1e-3) / 1e9,
+                     "hbm_frac": b / (ms / steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "kernels_per_step": info["kernels_per_step"]}
+        if segs:
+            out[prec]["segments"] = segs
+    del wl
+    return out
 
 
 def main():
@@ -135,6 +162,7 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32_k64", "fp64"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--accuracy-steps", type=int, default=1000)
+    ap.add_argument("--configs", default="c3,c4,c5", help="other workloads to time (comma list, or 'none')")
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for rocprofv3 counter passes: the fused stage on the workload "
                          "plus a 256 MiB device copy as the HBM byte-count calibration")
@@ -145,24 +173,21 @@ def main():
         sys.exit(2)
 
     import nekwave_amd as nw
+    from nekwave_amd import workloads as W
 
     if args.profile_run:
-        profile_run(nw, args.precision, steps=min(args.steps, 4))
+        profile_run(nw, W, args.precision, steps=min(args.steps, 4))
         return
 
     dev = nw.device_info(0)
-    m = build_mesh(nw)
-    info = m.info()
-    npts, nelt = info.npts, info.nelt
-    x, y, z = m.coords()
-    st0 = initial_state(x, y, z)
-    dt = m.computeAutomaticDt(1.0, 0.4)
+    wl = W.c2()
+    m = wl.meshes[0]
+    npts, nelt, dt = wl.npts, wl.nelt, wl.dt
     dof = 6 * npts
 
-    ms, wall, sinfo = time_path(nw, m, st0, dt, args.precision, "fused", args.steps, args.warmup)
+    ms, wall, sinfo, _ = time_path(nw, wl, args.precision, "fused", args.steps, args.warmup)
     value = dof * args.steps / (ms * 1e-3)
-    s_u = 8 if args.precision == "fp64" else 4
-    s_k = 4 if args.precision == "fp32" else 8
+    s_u, s_k = SIZES[args.precision]
     b_step = algorithmic_bytes_per_step(npts, nelt, s_u, s_k)
     launches = sinfo["kernels_per_step"] * args.steps
     per_launch_ms = ms / launches
@@ -171,8 +196,8 @@ def main():
 
     # comparison paths (fewer steps: they are slower)
     cmp_steps = max(10, args.steps // 5)
-    ms64, _, _ = time_path(nw, m, st0, dt, "fp64", "fused", cmp_steps, 5)
-    ms64p, _, _ = time_path(nw, m, st0, dt, "fp64", "plain", cmp_steps, 5)
+    ms64, _, _, _ = time_path(nw, wl, "fp64", "fused", cmp_steps, 5)
+    ms64p, _, _, _ = time_path(nw, wl, "fp64", "plain", cmp_steps, 5)
     v64 = dof * cmp_steps / (ms64 * 1e-3)
     v64p = dof * cmp_steps / (ms64p * 1e-3)
 
@@ -181,14 +206,17 @@ def main():
     if args.accuracy_steps > 0 and args.precision != "fp64":
         outs = {}
         for prec in (args.precision, "fp64"):
-            s = nw.DgSolver(precision=prec, pipeline="fused")
-            s.initialize(m, 1.0)
-            s.uploadState(st0)
+            s = make_solver(nw, wl, prec)
             s.run(dt, args.accuracy_steps)
             outs[prec] = s.downloadState()
             s.finalize()
         w = m.getJac() * np.tile(m.getW3(), nelt)  # dV = J w3 (3dboxpec_test.cpp:84)
         acc = rel_l2(outs[args.precision], outs["fp64"], w)
+
+    others = {}
+    if args.configs and args.configs != "none":
+        for name in args.configs.split(","):
+            others[name] = other_config(nw, W, name.strip(), 20)
 
     cpu = None if args.no_cpu else cpu_baseline(dt)
 
@@ -213,13 +241,14 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
-                     "kernel": "k_fused_stage<8,*,%s>" % args.precision,
+                     "kernel": "k_fused_stage<8,*,*,%s>" % args.precision,
                      "algorithmic_bytes_per_launch": b_step / 5, "mean_launch_ms": per_launch_ms},
         "fp64_fused": {"value": v64, "ms_per_step": ms64 / cmp_steps},
         "fp64_plain": {"value": v64p, "ms_per_step": ms64p / cmp_steps},
         "speedup_vs_fp64_plain": value / v64p,
         "rel_l2_vs_fp64": acc,
         "rel_l2_steps": args.accuracy_steps,
+        "other_configs": others,
         "wall_s_timed": wall,
         "cpu_baseline": cpu,
     }

@@ -203,6 +203,11 @@ int nw_launch_pack_halo(const double* state, double* sendBuf, const int* sendVol
 int nw_launch_unpack_halo(const double* recvBuf, double* state, const int* recvBufOffsets, const int* exchangeSizes,
                           int numHaloPoints, int64_t npts, int64_t stateStride, void* stream);
 
+/* Seeded synthetic inputs: std::mt19937_64(seed) with uniform = (x >> 11) 2^-53,
+ * normal = Box-Muller first variate of (1 - uniform, uniform); n draws. */
+int nw_rand_uniform(uint64_t seed, size_t n, double* out);
+int nw_rand_normal(uint64_t seed, size_t n, double* out);
+
 /* Device memory helpers for callers without their own allocator (tests, FFI). */
 int nw_dev_alloc(size_t bytes, void** p);
 int nw_dev_free(void* p);

@@ -90,6 +90,8 @@ SIGNATURES = {
     "nw_launch_lsrk45_update": (i, [vp, vp, vp, d, d, d, i64, i64, vp]),
     "nw_launch_pack_halo": (i, [vp, vp, vp, vp, vp, i, i64, vp]),
     "nw_launch_unpack_halo": (i, [vp, vp, vp, vp, i, i64, i64, vp]),
+    "nw_rand_uniform": (i, [C.c_uint64, sz, dp]),
+    "nw_rand_normal": (i, [C.c_uint64, sz, dp]),
     "nw_dev_alloc": (i, [sz, C.POINTER(vp)]),
     "nw_dev_free": (i, [vp]),
     "nw_dev_h2d": (i, [vp, vp, sz]),

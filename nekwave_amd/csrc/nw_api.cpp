@@ -2,6 +2,7 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -364,6 +365,24 @@ NW_API int nw_launch_pack_halo(const double* st, double* buf, const int* vol, co
 NW_API int nw_launch_unpack_halo(const double* buf, double* st, const int* off, const int* sizes, int nh,
                                  int64_t npts, int64_t S, void* stream) {
     NW_TRY_HIP(nw::launch_unpack<double>(buf, st, off, sizes, nh, npts, S, hs(stream)));
+    return NW_OK;
+}
+
+// ---------------------------------------------------------------- rng ----
+NW_API int nw_rand_uniform(uint64_t seed, size_t n, double* out) {
+    NW_CHECK_ARG(out || n == 0, "null out");
+    std::mt19937_64 g(seed);
+    for (size_t x = 0; x < n; ++x) out[x] = double(g() >> 11) * (1.0 / 9007199254740992.0);
+    return NW_OK;
+}
+NW_API int nw_rand_normal(uint64_t seed, size_t n, double* out) {
+    NW_CHECK_ARG(out || n == 0, "null out");
+    std::mt19937_64 g(seed);
+    for (size_t x = 0; x < n; ++x) {
+        const double u1 = 1.0 - double(g() >> 11) * (1.0 / 9007199254740992.0);
+        const double u2 = double(g() >> 11) * (1.0 / 9007199254740992.0);
+        out[x] = std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+    }
     return NW_OK;
 }
 
