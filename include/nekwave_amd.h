@@ -79,6 +79,22 @@ int nw_mesh_box(int n1, int nelx, int nely, int nelz, const double bounds[6], in
  * and must be conforming (one of the 8 square symmetries). */
 int nw_mesh_from_corners(int n1, int nelt, const double* corners, const int* face_kind, const int* nbr_elem,
                          const int* nbr_face, nw_mesh** out);
+/* Mesh::loadFromRea / Mesh::loadFromRe2 (mesh.cpp:317-636) as Case::loadMesh
+ * picks them (case.cpp:149-153): a path containing ".re2" is read as binary
+ * .re2, anything else as text .rea (NEL < 0 there redirects to the .re2 of the
+ * same stem).  Boundary tags follow the reference's reading: on .rea, "PEC" "W"
+ * "V" "v" are PEC, "PMC" "SYM" "S" "s" PMC (mirrored as PEC), other tags with a
+ * neighbour are matched by coordinates and faces that match nowhere are PEC, so
+ * a raw periodic "P" tag is PEC; flags NW_MESH_REA_PERIODIC pairs "P" faces by
+ * (p, q) identity instead.  On .re2, tags containing 'P' are periodic and faces
+ * without a record pair by centroid.  Inconsistent files (a face missing from
+ * the list, partly matching faces, curved-side records) fail with NW_ERR_MESH
+ * or NW_ERR_UNSUPPORTED where the reference would continue. */
+enum { NW_MESH_REA_PERIODIC = 1 };
+int nw_mesh_load(const char* path, int n1, int flags, nw_mesh** out);
+/* The face list the mesh was built from, nelt*6 each (e-major): kind
+ * (NW_FACE_*), neighbour element and face (-1 = none). */
+int nw_mesh_faces(const nw_mesh* m, int* face_kind, int* nbr_elem, int* nbr_face);
 /* Extension: per-element permittivity/permeability (NULL = 1).  The inverse
  * mass uses them (dg_kernels.cu:365-370 has eps = mu = 1) and the upwind flux
  * becomes impedance weighted; eps = mu = 1 reproduces the reference. */

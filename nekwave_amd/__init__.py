@@ -3,9 +3,9 @@
 Python mirror of the reference's host interface over the library's C ABI
 (include/nekwave_amd.h):
 
-    Mesh       -- reference Mesh (src/mesh.hpp:51-218): createBoxMesh, GLL data,
-                  coordinates, dt helpers; extensions for jitter, media and the
-                  single-GPU halo stand-in.
+    Mesh       -- reference Mesh (src/mesh.hpp:51-218): createBoxMesh,
+                  loadFromRea/loadFromRe2, GLL data, coordinates, dt helpers;
+                  extensions for jitter, media and the single-GPU halo stand-in.
     DgSolver   -- reference DgSolver (src/dg_solver.hpp:35-102): initialize,
                   uploadState, downloadState, step, finalize; extensions for
                   precision, pipeline, graphs, timed runs and device diagnostics.
@@ -15,6 +15,8 @@ Python mirror of the reference's host interface over the library's C ABI
 All numerics run in the HIP library; nothing here computes the step.
 """
 import ctypes as C
+import os
+import sys
 
 import numpy as np
 
@@ -80,6 +82,39 @@ class Mesh:
                                          C.byref(h)))
         m._h = h
         return m
+
+    def load(self, filename, periodic_p=False):
+        """Read a NekCEM mesh file (.re2 when the name contains ".re2", else .rea);
+        raises NekWaveError on failure."""
+        self._free()
+        h = C.c_void_p()
+        check(lib().nw_mesh_load(os.fsencode(filename), self._N,
+                                 _lib.NW_MESH_REA_PERIODIC if periodic_p else 0, C.byref(h)))
+        self._h = h
+        return True
+
+    def loadFromRea(self, filename, periodic_p=False):
+        """Mesh::loadFromRea (mesh.cpp:317-436): True on success; on failure the
+        message goes to stderr and False is returned, as in the reference."""
+        try:
+            return self.load(filename, periodic_p)
+        except NekWaveError as e:
+            print("Error: %s" % e, file=sys.stderr)
+            return False
+
+    def loadFromRe2(self, filename):
+        """Mesh::loadFromRe2 (mesh.cpp:439-636); True/False like loadFromRea."""
+        if ".re2" not in filename:
+            print("Error: %s is not a .re2 file name" % filename, file=sys.stderr)
+            return False
+        return self.loadFromRea(filename)
+
+    def getFaces(self):
+        """The face list (kind NW_FACE_*, neighbour element, neighbour face), nelt x 6 each."""
+        n = 6 * self.getNumElements()
+        k = np.zeros(n, np.int32); ne = np.zeros(n, np.int32); nf = np.zeros(n, np.int32)
+        check(lib().nw_mesh_faces(self.handle, _ip(k), _ip(ne), _ip(nf)))
+        return k.reshape(-1, 6), ne.reshape(-1, 6), nf.reshape(-1, 6)
 
     def setMaterials(self, eps=None, mu=None):
         e = None if eps is None else np.ascontiguousarray(eps, dtype=np.float64)

@@ -15,6 +15,7 @@ STATUS_NAMES = {0: "NW_OK", 1: "NW_ERR_ARG", 2: "NW_ERR_HIP", 3: "NW_ERR_ALLOC",
 PREC = {"fp64": 0, "fp32": 1, "fp32_k64": 2}
 PIPE = {"plain": 0, "fused": 1}
 FACE_E, FACE_PERIODIC, FACE_PEC, FACE_PMC = 0, 1, 2, 3
+NW_MESH_REA_PERIODIC = 1
 
 
 class NekWaveError(RuntimeError):
@@ -55,6 +56,8 @@ SIGNATURES = {
     "nw_device_info": (i, [i, ip, C.c_char_p, sz, ip, C.POINTER(sz)]),
     "nw_mesh_box": (i, [i, i, i, i, dp, i, i, i, d, C.c_uint64, C.POINTER(vp)]),
     "nw_mesh_from_corners": (i, [i, i, dp, ip, ip, ip, C.POINTER(vp)]),
+    "nw_mesh_load": (i, [C.c_char_p, i, i, C.POINTER(vp)]),
+    "nw_mesh_faces": (i, [vp, ip, ip, ip]),
     "nw_mesh_set_materials": (i, [vp, dp, dp]),
     "nw_mesh_self_halo_x": (i, [vp, ip]),
     "nw_mesh_get_info": (i, [vp, C.POINTER(MeshInfo)]),

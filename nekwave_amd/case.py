@@ -1,6 +1,6 @@
 """Case lifecycle over the solver (reference Case, src/case.hpp:48-153, case.cpp:94-728).
 
-preprocess(): box mesh, dt rule, initial condition hook evaluated at the nodes.
+preprocess(): box mesh or mesh file, dt rule, initial condition hook evaluated at the nodes.
 simulate():   DgSolver.initialize + uploadState, then the step loop with the
               reference's termination and last-dt clamp (case.cpp:562-583).
               Unlike the reference (which downloads the whole state every step,
@@ -10,11 +10,12 @@ postprocess(): user hook, then DgSolver.finalize.
 
 Config keeps the subset of the reference's keys that shape the step
 (config.hpp:21-254): order, cfl, dt, c0/flux_type, nelx/nely/nelz, bounds,
-periodic_*, num_steps/max_steps, final_time, output_frequency; plus the
+periodic_*, mesh file, num_steps/max_steps, final_time, output_frequency; plus the
 extensions precision, pipeline, jitter, seed.
 """
 import json
 import math
+import sys
 
 import numpy as np
 
@@ -31,7 +32,8 @@ class Config:
     _ALIASES = {"n": "order", "polynomial_order": "order", "num_steps": "num_steps", "nsteps": "num_steps",
                 "max_steps": "num_steps", "t_final": "final_time", "final_time": "final_time",
                 "output_frequency": "output_frequency", "outputfreq": "output_frequency",
-                "flux": "flux_type"}
+                "flux": "flux_type", "mesh": "mesh_file", "file": "mesh_file", "path": "mesh_file",
+                "meshfile": "mesh_file"}
 
     def __init__(self, **kw):
         self.order = 3
@@ -44,6 +46,7 @@ class Config:
         self.nelx = self.nely = self.nelz = 0
         self.xmin, self.xmax, self.ymin, self.ymax, self.zmin, self.zmax = -1.0, 1.0, -1.0, 1.0, -1.0, 1.0
         self.periodic_x = self.periodic_y = self.periodic_z = False
+        self.mesh_file = ""
         self.precision = "fp64"
         self.pipeline = "fused"
         self.jitter = 0.0
@@ -118,6 +121,10 @@ class Case:
         if c.nelx > 0 and c.nely > 0 and c.nelz > 0:
             self.mesh.createBoxMesh(c.nelx, c.nely, c.nelz, c.xmin, c.xmax, c.ymin, c.ymax, c.zmin, c.zmax,
                                     c.periodic_x, c.periodic_y, c.periodic_z, jitter=c.jitter, seed=c.seed)
+        elif c.mesh_file and c.mesh_file != "box":  # case.cpp:144-156
+            if not self.mesh.loadFromRea(c.mesh_file):
+                print("Warning: Could not load mesh. Falling back to default 3x3x3 mesh.", file=sys.stderr)
+                self.mesh.createBoxMesh(3, 3, 3, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0)
         else:  # the reference's default 3x3x3 box (case.cpp:158-161)
             self.mesh.createBoxMesh(3, 3, 3, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0)
         dxmin = self.mesh.computeMinNodeDistance()

@@ -87,6 +87,26 @@ NW_API int nw_mesh_from_corners(int n1, int nelt, const double* corners, const i
     NW_GUARD_END
 }
 
+NW_API int nw_mesh_load(const char* path, int n1, int flags, nw_mesh** out) {
+    NW_GUARD_BEGIN
+    NW_CHECK_ARG(out && path, "nw_mesh_load: null argument");
+    *out = nullptr;
+    int st = NW_OK;
+    nw::Mesh* m = nw::Mesh::load(path, n1, flags, &st);
+    return wrap_mesh(m, st, out);
+    NW_GUARD_END
+}
+
+NW_API int nw_mesh_faces(const nw_mesh* m, int* kind, int* ne, int* nf) {
+    NW_CHECK_ARG(m && kind && ne && nf, "null argument");
+    const size_t n = m->m.kind.size();
+    for (size_t x = 0; x < n; ++x)  // cut faces of the halo stand-in are periodic faces fed through ghosts
+        kind[x] = m->m.kind[x] == nw::NW_FACE_HALO_INTERNAL ? NW_FACE_PERIODIC : m->m.kind[x];
+    std::memcpy(ne, m->m.nbr_e.data(), sizeof(int) * n);
+    std::memcpy(nf, m->m.nbr_f.data(), sizeof(int) * n);
+    return NW_OK;
+}
+
 NW_API int nw_mesh_set_materials(nw_mesh* m, const double* eps, const double* mu) {
     NW_GUARD_BEGIN
     NW_CHECK_ARG(m, "null mesh");

@@ -170,7 +170,9 @@ int Mesh::finish() {
             const int k = kind[fi];
             fbc[fi] = (k == NW_FACE_PEC) ? 1 : (k == NW_FACE_PMC) ? 2 : 0;
             const int e2 = nbr_e[fi], f2 = nbr_f[fi];
-            if (k == NW_FACE_PEC || e2 < 0) {
+            // PEC and PMC faces both take the PEC mirror (dg_kernels.cu:244 tests
+            // bc != 0), whatever neighbour the face list names
+            if (k == NW_FACE_PEC || k == NW_FACE_PMC || e2 < 0) {
                 fcode[fi] = -1;
                 continue;
             }
@@ -202,6 +204,32 @@ int Mesh::finish() {
                         ok = dx * dx + dy * dy + dz * dz < 1e-10;
                     }
                 if (ok) found = o;
+            }
+            if (found < 0 && unmatched_is_pec) {
+                // mesh.cpp:1048-1070 matches node by node and leaves unmatched nodes
+                // at -1, which dg_solver.cpp:102-104 turns into PEC; a face none of
+                // whose nodes lies on the neighbour face is therefore a PEC face
+                bool any = false;
+                for (int q = 0; q < n1 && !any; ++q)
+                    for (int p = 0; p < n1 && !any; ++p) {
+                        int i, j, kk;
+                        face_ijk(f, n1, p, q, i, j, kk);
+                        double A[3];
+                        map_point(&corners[size_t(e) * 24], z[i], z[j], z[kk], A);
+                        for (int q2 = 0; q2 < n1 && !any; ++q2)
+                            for (int p2 = 0; p2 < n1 && !any; ++p2) {
+                                int i2, j2, k2;
+                                face_ijk(f2, n1, p2, q2, i2, j2, k2);
+                                double B[3];
+                                map_point(&corners[size_t(e2) * 24], z[i2], z[j2], z[k2], B);
+                                const double dx = A[0] - B[0], dy = A[1] - B[1], dz = A[2] - B[2];
+                                any = dx * dx + dy * dy + dz * dz < 1e-10;
+                            }
+                    }
+                if (!any) {
+                    fcode[fi] = -1;
+                    continue;
+                }
             }
             if (found < 0) {
                 set_error("non-conforming E face at element " + std::to_string(e) + " face " + std::to_string(f));

@@ -15,6 +15,7 @@ struct Mesh {
     bool box_conforming = false;
     bool all_affine = false;
     bool media = false;
+    bool unmatched_is_pec = false;               // E faces with no coincident node become PEC (file readers)
     std::vector<double> corners;                 // nelt * 24
     std::vector<int> kind, nbr_e, nbr_f;         // nelt * 6 (FaceInfo list)
     std::vector<int> fcode, fbc;                 // nelt * 6 (neighbour code, reference bc code)
@@ -31,6 +32,7 @@ struct Mesh {
                      double jitter, uint64_t seed, int* status);
     static Mesh* from_corners(int n1, int nelt, const double* corners, const int* kind, const int* ne,
                               const int* nf, int* status);
+    static Mesh* load(const char* path, int n1, int flags, int* status);  // nw_meshio.cpp
     int finish();
     int set_materials(const double* eps, const double* mu);
     int self_halo_x();

@@ -7,6 +7,7 @@
 //
 //   ref_mesh_dump box <n1> <nelx> <nely> <nelz> <xmin> <xmax> <ymin> <ymax> <zmin> <zmax> <px> <py> <pz> <out>
 //   ref_mesh_dump rea <n1> <file.rea> <out>
+//   ref_mesh_dump re2 <n1> <file.re2> <out>
 //
 // Output (little-endian): int32 n1, nelt, npts, tfp; then float64 z[n1], w[n1],
 // D[n1^2], Dt[n1^2], w3[n1^3], corners[nelt*24], x, y, z, jac, rx, sx, tx, ry,
@@ -37,6 +38,9 @@ int main(int argc, char** argv) {
         out = argv[15];
     } else if (mode == "rea" && argc == 5) {
         if (!mesh.loadFromRea(argv[3])) return 3;
+        out = argv[4];
+    } else if (mode == "re2" && argc == 5) {
+        if (!mesh.loadFromRe2(argv[3])) return 3;
         out = argv[4];
     } else {
         return 2;

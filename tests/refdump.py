@@ -86,3 +86,55 @@ def rea(tmpdir, n1, corners, kind, nbr_e, nbr_f, name="mesh"):
     write_rea(rp, corners, kind, nbr_e, nbr_f)
     subprocess.run([DUMPER, "rea", str(n1), rp, bp], check=True, stdout=subprocess.DEVNULL)
     return _read(bp)
+
+
+def write_rea_tags(path, corners, tags, nbr_e, nbr_f, nel_line=None):
+    """.rea text with raw boundary tags (one per face, e-major) and 1-based
+    neighbour columns (0 = none), as NekCEM writes them."""
+    corners = np.asarray(corners, dtype=np.float64).reshape(-1, 8, 3)
+    nelt = corners.shape[0]
+    lines = ["generated mesh", "**MESH DATA**", nel_line or ("%d 3 %d NEL,NDIM,NELV" % (nelt, nelt))]
+    for e in range(nelt):
+        lines.append("ELEMENT %d [ 1a] GROUP 0" % (e + 1))
+        c = corners[e]
+        for lo in (0, 4):
+            for d in range(3):
+                lines.append(" ".join(repr(float(c[lo + a, d])) for a in range(4)))
+    lines.append("***** CURVED SIDE DATA *****")
+    lines.append("***** BOUNDARY CONDITIONS *****")
+    lines.append("***** FLUID BOUNDARY CONDITIONS *****")
+    for fi in range(nelt * 6):
+        e, f = divmod(fi, 6)
+        ne = int(nbr_e[fi]) + 1 if nbr_e[fi] >= 0 else 0
+        nf = int(nbr_f[fi]) + 1 if nbr_f[fi] >= 0 else 0
+        lines.append(" %-3s %d %d %d.00000 %d.00000 0.00000 0.00000 0.00000" % (tags[fi], e + 1, f + 1, ne, nf))
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+
+
+def write_re2(path, corners, records, ncurv=0, nel_in_header=True):
+    """Binary .re2 in the layout Mesh::loadFromRe2 reads (mesh.cpp:439-636):
+    80-byte header, float32 6.54321, per element group + x[8] + y[8] + z[8]
+    (float64), ncurv, nbc, then per record e, f, ne, nf, p1, p2, p3 (float64,
+    1-based, 0 = none) and an 8-byte tag."""
+    corners = np.asarray(corners, dtype=np.float64).reshape(-1, 8, 3)
+    nelt = corners.shape[0]
+    hdr = ("#v002 %d 3 %d" % (nelt, nelt)) if nel_in_header else "#v002"
+    buf = [hdr.ljust(80).encode(), np.array([6.54321], np.float32).tobytes()]
+    for e in range(nelt):
+        buf.append(np.concatenate([[0.0], corners[e, :, 0], corners[e, :, 1], corners[e, :, 2]]).tobytes())
+    buf.append(np.array([float(ncurv), float(len(records))]).tobytes())
+    for (e, f, ne, nf, tag) in records:
+        buf.append(np.array([e + 1, f + 1, ne + 1 if ne >= 0 else 0, nf + 1 if nf >= 0 else 0, 0, 0, 0],
+                            np.float64).tobytes())
+        buf.append(tag.encode().ljust(8)[:8])
+    with open(path, "wb") as fh:
+        fh.write(b"".join(buf))
+
+
+def dump_file(tmpdir, mode, n1, mesh_path, name="dump"):
+    bp = os.path.join(str(tmpdir), name + ".bin")
+    r = subprocess.run([DUMPER, mode, str(n1), mesh_path, bp], stdout=subprocess.DEVNULL)
+    if r.returncode != 0:
+        return None
+    return _read(bp)

@@ -1,0 +1,177 @@
+"""Parity at the full workload sizes (nekwave_amd.workloads c2..c5).
+
+The oracle runs a few steps at the real shape on the host cores; longer runs
+are checked through properties that do not depend on size:
+  * c2 (16^3, n1 = 8, jittered periodic): FP64 fused and plain vs the oracle,
+    20 steps, <= 1e-12; FP32 vs the FP64 device path after 1000 steps <= 1e-5
+    (the transitive FP64 oracle; the FP64 path is pinned to the oracle above);
+  * c3 (32^3, n1 = 8, eps ~ U[1,4]): FP64 fused vs the oracle, 2 steps; upwind
+    energy non-increasing on the device over 50 FP32 steps;
+  * c4 (256 cavities): a spread of instances bitwise equal to their own
+    single-mesh runs, and within the FP32 bound of the oracle;
+  * c5 (64^3, n1 = 6, lognormal media, x cut faces fed by pack -> copy ->
+    unpack): FP64 vs the oracle, 1 step; the halo path bitwise equal to the
+    same mesh with periodic-x faces read in place.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import analytic as an
+import nekwave_amd as nw
+import pyoracle as po
+from nekwave_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+po.set_threads(min(16, os.cpu_count() or 1))
+
+
+def rel_l2(dV, a, b):
+    return an.rel_l2(dV, a, b)
+
+
+def product_run(mesh, st, dt, steps, precision="fp64", pipeline="fused", c0=1.0):
+    s = nw.DgSolver(precision=precision, pipeline=pipeline, use_graph=True)
+    s.initialize(mesh, c0)
+    s.uploadState(st)
+    if steps:
+        s.run(dt, steps)
+    out = s.downloadState()
+    s.finalize()
+    return out
+
+
+def oracle_run(o, st, dt, steps, c0=1.0):
+    s = po.Solver(o, c0)
+    s.upload(st)
+    s.step(dt, steps)
+    return s.download()
+
+
+@pytest.fixture(scope="module")
+def c2():
+    wl = W.c2()
+    o = po.Mesh.box(8, (16, 16, 16), (0, 1, 0, 1, 0, 1), (1, 1, 1), jitter=0.1, seed=2)
+    print("c2: nodes=%d dt=%.6e" % (wl.npts, wl.dt), flush=True)
+    ref = oracle_run(o, wl.states[0], wl.dt, 20)
+    return wl, o, ref
+
+
+@pytest.mark.parametrize("pipeline", ["fused", "plain"])
+def test_c2_fp64_matches_oracle(c2, pipeline):
+    wl, o, ref = c2
+    assert o.auto_dt() == pytest.approx(wl.dt, rel=1e-15)
+    got = product_run(wl.meshes[0], wl.states[0], wl.dt, 20, "fp64", pipeline)
+    err = rel_l2(an.dV(o), got, ref)
+    print("c2 fp64 %s 20 steps relL2=%.3e" % (pipeline, err), flush=True)
+    assert err <= 1e-12, err
+
+
+@pytest.mark.parametrize("precision,bound", [("fp32", 1e-5), ("fp32_k64", 1e-5)])
+def test_c2_fp32_vs_fp64_after_1000_steps(c2, precision, bound):
+    wl, o, _ = c2
+    ref = product_run(wl.meshes[0], wl.states[0], wl.dt, 1000, "fp64", "fused")
+    got = product_run(wl.meshes[0], wl.states[0], wl.dt, 1000, precision, "fused")
+    err = rel_l2(an.dV(o), got, ref)
+    print("c2 %s vs fp64 after 1000 steps relL2=%.3e" % (precision, err), flush=True)
+    assert err <= bound, err
+
+
+def _c3_oracle(wl):
+    o = po.Mesh.box(8, (32, 32, 32), (0, 1, 0, 1, 0, 1))
+    o.set_materials(1.0 + 3.0 * W.uniforms(3, 32 ** 3), np.ones(32 ** 3))
+    return o
+
+
+def test_c3_fp64_matches_oracle():
+    wl = W.c3()
+    o = _c3_oracle(wl)
+    assert o.auto_dt() == pytest.approx(wl.dt, rel=1e-15)
+    ref = oracle_run(o, wl.states[0], wl.dt, 2)
+    got = product_run(wl.meshes[0], wl.states[0], wl.dt, 2, "fp64", "fused")
+    err = rel_l2(an.dV(o), got, ref)
+    print("c3 fp64 fused 2 steps relL2=%.3e" % err, flush=True)
+    assert err <= 1e-12, err
+
+
+def test_c3_upwind_energy_nonincreasing_fp32():
+    wl = W.c3()
+    e_ref = _c3_oracle(wl).energy(wl.states[0])
+    s = nw.DgSolver(precision="fp32", pipeline="fused", use_graph=True)
+    s.initialize(wl.meshes[0], 1.0)
+    s.uploadState(wl.states[0])
+    e = [s.diagnostics()["energy"]]
+    for _ in range(5):
+        s.run(wl.dt, 10)
+        e.append(s.diagnostics()["energy"])
+    s.finalize()
+    print("c3 energies", e, flush=True)
+    assert e[0] == pytest.approx(e_ref, rel=1e-6)
+    for a, b in zip(e, e[1:]):
+        assert b <= a * (1 + 1e-6)
+
+
+def test_c4_instances_match_single_runs_and_oracle():
+    wl = W.c4()
+    b = nw.BatchSolver(precision="fp32", use_graph=True)
+    b.initialize(wl.meshes, 1.0)
+    for x, st in enumerate(wl.states):
+        b.uploadState(x, st)
+    b.run(wl.dt, 10)
+    spec = wl.info["instances"]
+    picks = {}
+    for x, (k, p, eps, mm, nn, ph) in enumerate(spec):
+        picks.setdefault(p, x)  # the first instance of every order
+    assert len(picks) == 6
+    for p, x in sorted(picks.items()):
+        k, _, eps, mm, nn, ph = spec[x]
+        got = b.downloadState(x)
+        alone = product_run(wl.meshes[x], wl.states[x], wl.dt, 10, "fp32", "fused")
+        assert np.array_equal(got, alone), x
+        o = po.Mesh.box(p + 1, (k, k, k), (0, 1, 0, 1, 0, 1))
+        o.set_materials(np.full(k ** 3, eps), np.ones(k ** 3))
+        ref = oracle_run(o, wl.states[x], wl.dt, 10)
+        err = rel_l2(an.dV(o), got, ref)
+        print("c4 instance %d (k=%d p=%d) relL2=%.3e" % (x, k, p, err), flush=True)
+        assert err <= 2e-5, (x, err)
+    b.finalize()
+
+
+def _c5_media(nel=64):
+    """The c5 draws restated: seed 5, 6 pulse uniforms, then 4 per element."""
+    u = W.uniforms(5, 6 + 4 * nel ** 3)[6:].reshape(-1, 4)
+    bm = lambda a, b: np.sqrt(-2.0 * np.log(1.0 - a)) * np.cos(2.0 * np.pi * b)  # noqa: E731
+    return np.exp(0.3 * bm(u[:, 0], u[:, 1])), np.exp(0.3 * bm(u[:, 2], u[:, 3]))
+
+
+def _c5_oracle():
+    o = po.Mesh.box(6, (64,) * 3, (0, 1, 0, 1, 0, 1), (1, 0, 0))
+    o.set_materials(*_c5_media())
+    o.self_halo_x()
+    return o
+
+
+def test_c5_fp64_matches_oracle():
+    wl = W.c5()
+    o = _c5_oracle()
+    assert o.auto_dt() == pytest.approx(wl.dt, rel=1e-15)
+    ref = oracle_run(o, wl.states[0], wl.dt, 1)
+    got = product_run(wl.meshes[0], wl.states[0], wl.dt, 1, "fp64", "fused")
+    err = rel_l2(an.dV(o), got, ref)
+    print("c5 fp64 fused 1 step relL2=%.3e" % err, flush=True)
+    assert err <= 1e-12, err
+
+
+def test_c5_halo_equals_periodic_in_place():
+    wl = W.c5()
+    m = wl.meshes[0]
+    eps, mu = _c5_media()
+    per = nw.Mesh(6)
+    per.createBoxMesh(64, 64, 64, 0, 1, 0, 1, 0, 1, True, False, False)
+    per.setMaterials(eps, mu)
+    for prec in ("fp32", "fp64"):
+        a = product_run(m, wl.states[0], wl.dt, 5, prec, "fused")
+        b = product_run(per, wl.states[0], wl.dt, 5, prec, "fused")
+        assert np.array_equal(a, b), prec
