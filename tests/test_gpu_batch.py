@@ -87,5 +87,10 @@ def test_batch_eigenmodes_match_closed_form():
         o = oracles[x]
         ex = an.pack(an.tm_mode(o.x, o.y, nst * dt, m=mn[0], n=mn[1], eps=eps, phase=ph))
         err = an.rel_l2(an.dV(o), b.downloadState(x), ex)
-        assert err < 0.02 * (mn[0] + mn[1]) / min(k, n1 - 1), (x, err)
+        # the discretisation error is the oracle's own (FP64) error to the closed
+        # form; a wrong eps scaling of omega would be a phase error of O(1)
+        err_o = an.rel_l2(an.dV(o), run_oracle(o, ics[x], dt, nst), ex)
+        print("instance %d n1=%d k=%d eps=%.1f mn=%s: closed-form error %.3e (oracle %.3e)"
+              % (x, n1, k, eps, mn, err, err_o))
+        assert abs(err - err_o) <= 1e-5 and err < 0.1, (x, err, err_o)
     b.finalize()
