@@ -100,7 +100,7 @@ def cpu_baseline(dt, budget_s=15.0):
     s = po.Solver(o, 1.0)
     s.upload(st)
     t1 = s.run(dt, 1)  # one step sizes the sample
-    n = max(1, min(40, int(budget_s / max(t1, 1e-3))))
+    n = max(1, min(1000, int(budget_s / max(t1, 1e-3))))
     t = s.run(dt, n)
     cpu_model = ""
     try:

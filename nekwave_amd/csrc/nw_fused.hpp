@@ -20,7 +20,9 @@ struct FusedArgs {
     long long ghost_base;     // first ghost slot (node index) of this segment
     int nelt;                 // elements in this segment
     int elem_base;            // global index of the segment's first element (into mat)
-    int nchunk;               // blocks per XCD chunk (XCD-aware block remap)
+    int ngroups;              // element groups (EPW elements each); set by the launcher
+    int gper;                 // groups per workgroup (persistent loop)
+    int wchunk;               // workgroups per XCD (XCD-aware remap)
     const ElemGeom* geom;     // per element, segment-local
     const ElemMat* mat;       // per element, global index (always present; vacuum = 1)
     const ElemMat* gmat;      // per ghost face
