@@ -77,228 +77,378 @@ __device__ __forceinline__ void pin(T& v) {
     asm volatile("" : "+v"(v));
 }
 
-// Occupancy target: enough waves per SIMD to cover HBM latency with a whole
-// stage's reads in flight per wave.
+// Occupancy target (waves per SIMD) that the register budget is sized for.
+// A workgroup of two waves per SIMD gets 4 (two workgroups per CU in 128
+// VGPRs); any other shape gets 3 (168 VGPRs), which keeps the prefetch
+// registers free of spills: a spill reload inside the loop would wait on every
+// prefetch in flight.
 template <int NT>
 struct WavesPerEU {
-    static constexpr int value = NT >= 512 ? 4 : 3;
+    static constexpr int value = ((NT + 63) / 64 + 3) / 4 == 2 ? 4 : 3;
 };
 
+// Global traffic goes through raw buffer instructions: the base address lives
+// in a buffer resource (SGPRs), each lane supplies a 32-bit byte offset and the
+// component stride rides in the scalar offset, so no lane computes a 64-bit
+// address.  Offsets past num_records read 0 instead of faulting.  The
+// launcher checks that every offset fits in 32 bits.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p) {
+    // dword3 0x00020000: the raw-buffer format word of the gfx9 family
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, 0xffffffffu, 0x00020000);
+}
+__device__ __forceinline__ float bload(float*, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+}
+__device__ __forceinline__ double bload(double*, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+}
+__device__ __forceinline__ void bstore(float v, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vo, so, 0);
+}
+__device__ __forceinline__ void bstore(double v, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, vo,
+                                          so, 0);
+}
+template <typename T>
+__device__ __forceinline__ T bld(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+    return bload(static_cast<T*>(nullptr), r, vo, so);
+}
+__device__ __forceinline__ int4 bld_int4(__amdgpu_buffer_rsrc_t r, unsigned vo) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, vo, 0, 0);
+    return make_int4(int(v[0]), int(v[1]), int(v[2]), int(v[3]));
+}
+
+// 1/J of the FP64 Jacobian: in FP32 the hardware reciprocal (1 ulp) of the
+// rounded determinant; in FP64 the IEEE quotient.
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+// |N| and 1/|N| from |N|^2 with the reference's guard |N| > 1e-14
+// (mesh.cpp:1032-1035): FP32 through the hardware reciprocal square root,
+// FP64 through the IEEE square root and quotient.
+__device__ __forceinline__ void norm_and_inverse(float n2, float& len, float& inv) {
+    inv = n2 > 1e-28f ? __builtin_amdgcn_rsqf(n2) : 0.0f;
+    len = n2 * inv;
+}
+__device__ __forceinline__ void norm_and_inverse(double n2, double& len, double& inv) {
+    len = sqrt(n2);
+    inv = len > 1e-14 ? 1.0 / len : 0.0;
+}
+
+// One workgroup owns a contiguous run of element groups (EPW elements each)
+// and walks it, one group per iteration; HBM reads are issued one or two
+// iterations before they are needed, so they travel while earlier groups
+// compute.  Per group:
+//   1. FP64 geometry of the thread's node; the node's face (Nanson) vectors
+//      and its field values go to LDS;                              barrier
+//   2. volume: the three reference derivatives of each field from LDS folded
+//      straight into J * curl;
+//   3. face point (f, p, q): jumps, flux, dA -> LDS;                barrier
+//   4. lift: each node gathers its (at most three) face terms, one per
+//      direction, in fixed face order; inverse mass; LSRK update; stores.
 template <int N1, int EPW, bool FIRST, typename R, typename KR>
 __global__ __launch_bounds__(EPW* N1* N1* N1) __attribute__((amdgpu_waves_per_eu(WavesPerEU<EPW * N1 * N1 * N1>::value)))
 void k_fused_stage(FusedArgs A) {
     constexpr int N2 = N1 * N1, NP = N1 * N2, NF = 6 * N2, NT = EPW * NP;
     constexpr int FPT = (EPW * NF + NT - 1) / NT;  // face points per thread (1 for n1 >= 6)
+    constexpr int ERW = 26;                         // doubles per element record: geometry (24), 1/eps, 1/mu
+    constexpr int GQ = (EPW * ERW + NT - 1) / NT;   // element-record words per thread
+    struct ElemRec {
+        ElemGeom g;
+        double ieps, imu;
+    };
+    static_assert(sizeof(ElemRec) == ERW * sizeof(double), "element record layout");
     __shared__ R sU[EPW][6][NP];
-    __shared__ R sF[EPW][NF][6];
-    __shared__ ElemGeom sG[EPW];
+    __shared__ R sF[EPW * NF + 1][6];  // face terms dA*F; the last row stays zero
+    __shared__ R sC[EPW * NF][9];      // per face point: Nanson vector, then the minus-side trace (6)
+    __shared__ R sD[N2];               // D, row-major: sD[i * N1 + l] = D_il
+    __shared__ ElemRec sE[2][EPW];     // element records, double-buffered over groups
 
     // XCD-aware remap: the hardware deals workgroups round-robin over the 8
-    // XCDs; each XCD gets a contiguous run of elements so face neighbours are
-    // read from its own L2.
+    // XCDs; logical workgroups, and with them element runs, are contiguous
+    // per XCD so face neighbours are read from that XCD's L2.
     const int hb = blockIdx.x;
-    const int L = (hb & 7) * A.nchunk + (hb >> 3);
-    if (L * EPW >= A.nelt) return;  // uniform over the block, before any barrier
+    const int W = (hb & 7) * A.wchunk + (hb >> 3);
+    const int g0 = W * A.gper;
+    const int g1 = min(g0 + A.gper, A.ngroups);
+    if (g0 >= g1) return;  // uniform over the block, before any barrier
 
     const int tid = threadIdx.x;
     const int es = tid / NP;
     const int m = tid - es * NP;
     const int i = m % N1, j = (m / N1) % N1, k = m / N2;
-    const int e = L * EPW + es;
-    const bool live = e < A.nelt;
-    int ee = live ? e : A.nelt - 1;  // clamped: every read stays in bounds
-    if (EPW == 1) ee = __builtin_amdgcn_readfirstlane(ee);
-    const R* __restrict__ uin = static_cast<const R*>(A.uin);
-    R* __restrict__ uout = static_cast<R*>(A.uout);
-    KR* __restrict__ kb = static_cast<KR*>(A.kbuf);
-    const long long S = A.stride, KS = A.kstride;
-    const long long node = A.node_base + (long long)ee * NP + m;
+    const __amdgpu_buffer_rsrc_t rU = make_rsrc(A.uin), rO = make_rsrc(A.uout), rK = make_rsrc(A.kbuf);
+    const __amdgpu_buffer_rsrc_t rW = make_rsrc(A.fw), rN = make_rsrc(A.fnb);
+    const __amdgpu_buffer_rsrc_t rR = make_rsrc(A.rec);
+    const unsigned sS = unsigned(A.stride) * unsigned(sizeof(R));    // component stride of u, bytes
+    const unsigned sK = unsigned(A.kstride) * unsigned(sizeof(KR));  // component stride of k, bytes
 
-    // ---- every HBM read of the stage is issued before the first barrier ----
-    R u[6];
-#pragma unroll
-    for (int c = 0; c < 6; ++c) u[c] = uin[c * S + node];
-    KR kold[6];
-#pragma unroll
-    for (int c = 0; c < 6; ++c) kold[c] = FIRST ? KR(0) : kb[c * KS + node];
-    R up[FPT][6];
-    int4 nbr[FPT];
+    // loop-invariant: node coordinates, inverse mass weight
+    const double zr = c_gll.z[N1][i], zs = c_gll.z[N1][j], zt = c_gll.z[N1][k];
+    const R Winv = R(c_gll.iw[N1][i] * c_gll.iw[N1][j] * c_gll.iw[N1][k]);
+    R wpq[FPT];  // w_p w_q of the thread's face point(s)
 #pragma unroll
     for (int t = 0; t < FPT; ++t) {
-        const int fp = tid + t * NT;
-        const int efs = fp / NF, r = fp - efs * NF;
-        const int f = r / N2, pq = r - f * N2, p = pq % N1, q = pq / N1;
-        const int ef = (fp < EPW * NF) ? min(L * EPW + efs, A.nelt - 1) : ee;
-        const int4 nb = A.fnb[ef * 6 + f];
-        nbr[t] = nb;
-        const long long idx = nb.x >= 0 ? (long long)nb.x + p * nb.y + q * nb.z : node;
-#pragma unroll
-        for (int c = 0; c < 6; ++c) up[t][c] = uin[c * S + idx];
+        const int r = (tid + t * NT) % NF, pq = r % N2;
+        wpq[t] = R(c_gll.w[N1][pq % N1] * c_gll.w[N1][pq / N1]);
     }
-    // rows of D for this node (constant memory), loaded with the fields
-    R Di[N1], Dj[N1], Dk[N1];
-#pragma unroll
-    for (int q = 0; q < N1; ++q) {
-        Di[q] = gD<R>(N1, i * N1 + q);
-        Dj[q] = gD<R>(N1, j * N1 + q);
-        Dk[q] = gD<R>(N1, k * N1 + q);
-    }
-    if (EPW > 1) {
-        for (int x = tid; x < EPW * 24; x += NT) {
-            const int el = min(L * EPW + x / 24, A.nelt - 1);
-            (&sG[0].c[0][0])[x] = (&A.geom[el].c[0][0])[x % 24];
-        }
-        __syncthreads();
-    }
+    const double dt = *A.dt;  // read once: a load inside the loop would wait on the prefetch
+    for (int x = tid; x < N2; x += NT) sD[x] = gD<R>(N1, x);
+    for (int x = tid; x < 6; x += NT) sF[EPW * NF][x] = R(0);
 
-    // ---- FP64 geometry while the reads are in flight ----
-    // volume: cofactor matrix C = J J^-T (mesh.cpp:291-309 times J)
-    R Crx, Cry, Crz, Csx, Csy, Csz, Ctx, Cty, Ctz, iJ;
+    auto clamp_e = [&](int g, int slot) { return min(g * EPW + slot, A.nelt - 1); };
+    auto node_of = [&](int g) -> unsigned {  // clamped: every read stays in bounds
+        return static_cast<unsigned>(A.node_base + (long long)clamp_e(g, es) * NP + m);
+    };
+    // Every read an iteration depends on is issued ahead of it, so no wave
+    // waits on memory latency inside an iteration:
+    //   two groups ahead: the face-neighbour table rows (they address the
+    //   neighbour-trace gather) and the element records (staged into LDS one
+    //   iteration ahead);
+    //   one group ahead: the fields and the neighbour traces;
+    //   at the top of the iteration: LSRK registers and flux weights, needed
+    //   only at its end.
+    struct Nbr {
+        int4 b[FPT];
+    };
+    struct Rec {
+        double w[GQ];
+    };
+    struct Pre {
+        R u[6];          // this node's fields
+        R up[FPT][6];    // plus-side trace of each face point (unused on mirror faces)
+        bool mirror[FPT];
+    };
+    auto fetch_nbr = [&](int g, Nbr& N) {
+#pragma unroll
+        for (int t = 0; t < FPT; ++t) {
+            const int fp = tid + t * NT;
+            const int efs = min(fp / NF, EPW - 1), f = (fp - (fp / NF) * NF) / N2;
+            N.b[t] = bld_int4(rN, unsigned(clamp_e(g, efs) * 6 + f) * 16u);
+        }
+    };
+    auto fetch_rec = [&](int g, Rec& Q) {
+#pragma unroll
+        for (int t = 0; t < GQ; ++t) {
+            const int x = min(tid + t * NT, EPW * ERW - 1);  // word x of the group's records
+            const unsigned el = unsigned(clamp_e(g, x / ERW));
+            Q.w[t] = bld<double>(rR, (el * unsigned(ERW) + unsigned(x - (x / ERW) * ERW)) * 8u, 0);
+        }
+    };
+    auto put_rec = [&](const Rec& Q, int buf) {
+#pragma unroll
+        for (int t = 0; t < GQ; ++t) {
+            const int x = tid + t * NT;
+            if (x < EPW * ERW) (&sE[buf][0].g.c[0][0])[x] = Q.w[t];
+        }
+    };
+    auto fetch = [&](int g, const Nbr& N, Pre& P) {
+        const unsigned nd = node_of(g);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) P.u[c] = bld<R>(rU, nd * unsigned(sizeof(R)), c * sS);
+        // issued by every thread (threads without a face point read their own
+        // node): the same number of loads on every path keeps the compiler's
+        // memory-counter waits exact instead of conservative
+#pragma unroll
+        for (int t = 0; t < FPT; ++t) {
+            const int fp = tid + t * NT;
+            const int r = fp - (fp / NF) * NF;
+            const int pq = r - (r / N2) * N2, p = pq % N1, q = pq / N1;
+            const int4 b = N.b[t];
+            P.mirror[t] = b.x < 0;
+            const unsigned idx = (b.x >= 0 && fp < EPW * NF) ? unsigned(b.x + p * b.y + q * b.z) : nd;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) P.up[t][c] = bld<R>(rU, idx * unsigned(sizeof(R)), c * sS);
+        }
+    };
+
+    // prologue: records of g0 in LDS, fields of g0 in flight, two-ahead reads of g0 + 1
+    Pre cur;
+    Nbr nb1;
+    Rec rec1;
     {
-        const Tangents T = tangents((EPW == 1) ? A.geom + ee : sG + es, c_gll.z[N1][i], c_gll.z[N1][j], c_gll.z[N1][k]);
-        double Cr[3], Cs[3], Ct[3];
-        cross3(T.s, T.t, Cr);  // (rx, ry, rz) * J
-        cross3(T.t, T.r, Cs);  // (sx, sy, sz) * J
-        cross3(T.r, T.s, Ct);  // (tx, ty, tz) * J
-        const double det = T.r[0] * Cr[0] + T.r[1] * Cr[1] + T.r[2] * Cr[2];
-        Crx = R(Cr[0]); Cry = R(Cr[1]); Crz = R(Cr[2]);
-        Csx = R(Cs[0]); Csy = R(Cs[1]); Csz = R(Cs[2]);
-        Ctx = R(Ct[0]); Cty = R(Ct[1]); Ctz = R(Ct[2]);
-        iJ = R(1.0 / det);
+        Nbr nb0;
+        Rec rec0;
+        fetch_nbr(g0, nb0);
+        fetch_rec(g0, rec0);
+        fetch_nbr(min(g0 + 1, g1 - 1), nb1);
+        fetch_rec(min(g0 + 1, g1 - 1), rec1);
+        put_rec(rec0, g0 & 1);
+        fetch(g0, nb0, cur);
     }
-    // computed now, while the reads are in flight, not after the barrier
-    pin(Crx); pin(Cry); pin(Crz); pin(Csx); pin(Csy); pin(Csz); pin(Ctx); pin(Cty); pin(Ctz); pin(iJ);
-    // faces: Nanson vector (mesh.cpp:1011-1038) = +-(a x b) of the face's two
-    // tangents: r-faces ds x dt, s-faces dt x dr, t-faces dr x ds
-    R fn[FPT][4];  // nx, ny, nz, dA
-#pragma unroll
-    for (int t = 0; t < FPT; ++t) {
-        const int fp = tid + t * NT;
-        const int efs = min(fp / NF, EPW - 1), r = fp - (fp / NF) * NF;
-        const int f = r / N2, pq = r - f * N2, p = pq % N1, q = pq / N1;
-        const int ml = face_local_node<N1>(f, p, q);
-        const int fi = ml % N1, fj = (ml / N1) % N1, fk = ml / N2;
-        const Tangents T = tangents((EPW == 1) ? A.geom + ee : sG + efs, c_gll.z[N1][fi], c_gll.z[N1][fj], c_gll.z[N1][fk]);
-        const bool rf = (f == 1 || f == 3), sf = (f == 0 || f == 2);
-        double a3[3], b3[3], Nv[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            a3[d] = rf ? T.s[d] : sf ? T.t[d] : T.r[d];
-            b3[d] = rf ? T.t[d] : sf ? T.r[d] : T.s[d];
-        }
-        cross3(a3, b3, Nv);
-        const double sg = (f == 1 || f == 2 || f == 5) ? 1.0 : -1.0;
-        const double Ln = sqrt(Nv[0] * Nv[0] + Nv[1] * Nv[1] + Nv[2] * Nv[2]);
-        const double il = Ln > 1e-14 ? sg / Ln : 0.0;
-        fn[t][0] = R(Nv[0] * il); fn[t][1] = R(Nv[1] * il); fn[t][2] = R(Nv[2] * il);
-        fn[t][3] = R(Ln * c_gll.w[N1][p] * c_gll.w[N1][q]);
-        pin(fn[t][0]); pin(fn[t][1]); pin(fn[t][2]); pin(fn[t][3]);
-    }
-
-#pragma unroll
-    for (int c = 0; c < 6; ++c) sU[es][c][m] = u[c];
     __syncthreads();
 
-    // ---- face terms dA * F of this thread's face point(s) -> LDS ----
+    for (int g = g0; g < g1; ++g) {
+        Nbr nb2;
+        Rec rec2;
+        fetch_nbr(min(g + 2, g1 - 1), nb2);
+        fetch_rec(min(g + 2, g1 - 1), rec2);
+        // reads of this group needed late (LSRK registers, flux weights) go
+        // before the prefetch: waiting on them does not wait on it
+        const unsigned nd = node_of(g);
+        KR kold[6];
 #pragma unroll
-    for (int t = 0; t < FPT; ++t) {
-        const int fp = tid + t * NT;
-        if (fp < EPW * NF) {
-            const int efs = fp / NF, r = fp - efs * NF;
-            const int f = r / N2, pq = r - f * N2, p = pq % N1, q = pq / N1;
-            const int ef = min(L * EPW + efs, A.nelt - 1);
-            const int ml = face_local_node<N1>(f, p, q);
-            const int4 nb = nbr[t];
-            const ElemMat Mm = A.mat[A.elem_base + ef];
-            const ElemMat Mp = nb.x < 0 ? Mm : (nb.w >= 0 ? A.mat[nb.w] : A.gmat[-(nb.w + 1)]);
-            R jE[3], jH[3];
-            if (nb.x < 0) {  // PEC mirror: [[E]] = -2 E-, [[H]] = 0 (dg_kernels.cu:244-251)
+        for (int c = 0; c < 6; ++c) kold[c] = FIRST ? KR(0) : bld<KR>(rK, nd * unsigned(sizeof(KR)), c * sK);
+        R fw[FPT][4];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) { jE[c] = R(-2) * sU[efs][c][ml]; jH[c] = R(0); }
-            } else {
+        for (int t = 0; t < FPT; ++t) {
+            const int fp = tid + t * NT;
+            const int efs = min(fp / NF, EPW - 1), f = (fp - (fp / NF) * NF) / N2;
+            const unsigned face = unsigned(clamp_e(g, efs) * 6 + f);
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    jE[c] = up[t][c] - sU[efs][c][ml];
-                    jH[c] = up[t][c + 3] - sU[efs][c + 3][ml];
+            for (int c = 0; c < 4; ++c) fw[t][c] = bld<R>(rW, (face * 4u + unsigned(c)) * unsigned(sizeof(R)), 0);
+        }
+        Pre nxt;
+        fetch(min(g + 1, g1 - 1), nb1, nxt);  // in flight while this group computes
+
+        const ElemRec& ER = sE[g & 1][es];
+
+        // ---- 1. FP64 geometry: cofactors C = J J^-T (mesh.cpp:291-309 times J) ----
+        R Crx, Cry, Crz, Csx, Csy, Csz, Ctx, Cty, Ctz, iJ;
+        {
+            const Tangents T = tangents(&ER.g, zr, zs, zt);
+            double Cr[3], Cs[3], Ct[3];
+            cross3(T.s, T.t, Cr);  // (rx, ry, rz) * J
+            cross3(T.t, T.r, Cs);  // (sx, sy, sz) * J
+            cross3(T.r, T.s, Ct);  // (tx, ty, tz) * J
+            const double det = T.r[0] * Cr[0] + T.r[1] * Cr[1] + T.r[2] * Cr[2];
+            Crx = R(Cr[0]); Cry = R(Cr[1]); Crz = R(Cr[2]);
+            Csx = R(Cs[0]); Csy = R(Cs[1]); Csz = R(Cs[2]);
+            Ctx = R(Ct[0]); Cty = R(Ct[1]); Ctz = R(Ct[2]);
+            iJ = recip(R(det));
+        }
+        // The Nanson vector of a face point is the cofactor row of its face's
+        // direction at that node (mesh.cpp:1011-1038: -+J (rx, ry, rz) etc.),
+        // outward: r-faces +-C_r, s-faces +-C_s, t-faces +-C_t.
+        // The node also hands its field values (the face point's minus-side
+        // trace) to the face point: rows of 9 values are read without LDS bank
+        // conflicts, which strided reads of the face nodes out of sU are not.
+        auto put = [&](int f, int pq, R x, R y, R z) {
+            R* o = sC[es * NF + f * N2 + pq];
+            o[0] = x; o[1] = y; o[2] = z;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) o[3 + c] = cur.u[c];
+        };
+        if (j == 0) put(0, i + N1 * k, -Csx, -Csy, -Csz);
+        if (i == N1 - 1) put(1, j + N1 * k, Crx, Cry, Crz);
+        if (j == N1 - 1) put(2, i + N1 * k, Csx, Csy, Csz);
+        if (i == 0) put(3, j + N1 * k, -Crx, -Cry, -Crz);
+        if (k == 0) put(4, i + N1 * j, -Ctx, -Cty, -Ctz);
+        if (k == N1 - 1) put(5, i + N1 * j, Ctx, Cty, Ctz);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) sU[es][c][m] = cur.u[c];
+        __syncthreads();
+        put_rec(rec1, (g + 1) & 1);  // read from the next iteration on, after the barrier below
+
+        // ---- 2. volume: reference derivatives (local_grad3, dg_kernels.cu:101-124)
+        // folded straight into J * curl (dg_kernels.cu:140-149) ----
+        R cE[3] = {R(0), R(0), R(0)}, cH[3] = {R(0), R(0), R(0)};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {  // reference direction r, s, t: one row of D live at a time
+            const int row = d == 0 ? i : (d == 1 ? j : k);
+            R Dr[N1];
+#pragma unroll
+            for (int q = 0; q < N1; ++q) Dr[q] = sD[row * N1 + q];
+            const R Cx = d == 0 ? Crx : (d == 1 ? Csx : Ctx);
+            const R Cy = d == 0 ? Cry : (d == 1 ? Csy : Cty);
+            const R Cz = d == 0 ? Crz : (d == 1 ? Csz : Ctz);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                R a = R(0);
+#pragma unroll
+                for (int q = 0; q < N1; ++q)
+                    a += Dr[q] * sU[es][c][d == 0 ? q + N1 * j + N2 * k : (d == 1 ? i + N1 * q + N2 * k : i + N1 * j + N2 * q)];
+                // J d/dx_e of field c gets a * C_d,e; curl u = (dy u_z - dz u_y, dz u_x - dx u_z, dx u_y - dy u_x)
+                R* cv = c < 3 ? cE : cH;
+                switch (c % 3) {
+                    case 0: cv[1] += a * Cz; cv[2] -= a * Cy; break;
+                    case 1: cv[0] -= a * Cz; cv[2] += a * Cx; break;
+                    default: cv[0] += a * Cy; cv[1] -= a * Cx; break;
                 }
+                // keep each contraction here: sinking it into the update below
+                // would hold the fields' LDS values in registers
+                pin(cE[0]); pin(cE[1]); pin(cE[2]); pin(cH[0]); pin(cH[1]); pin(cH[2]);
             }
-            const R nx = fn[t][0], ny = fn[t][1], nz = fn[t][2], dA = fn[t][3];
-            // central / upwind flux (dg_kernels.cu:265-294), impedance weighted with media
-            const R nxEx = ny * jE[2] - nz * jE[1], nxEy = nz * jE[0] - nx * jE[2], nxEz = nx * jE[1] - ny * jE[0];
-            const R nxHx = ny * jH[2] - nz * jH[1], nxHy = nz * jH[0] - nx * jH[2], nxHz = nx * jH[1] - ny * jH[0];
-            const R nnEx = ny * nxEz - nz * nxEy, nnEy = nz * nxEx - nx * nxEz, nnEz = nx * nxEy - ny * nxEx;
-            const R nnHx = ny * nxHz - nz * nxHy, nnHy = nz * nxHx - nx * nxHz, nnHz = nx * nxHy - ny * nxHx;
-            R wEh, wEe, wHe, wHh;  // F_E = wEh nx[[H]] - wEe nxnx[[E]];  F_H = -wHe nx[[E]] - wHh nxnx[[H]]
-            if (A.weighted) {
-                const double iz = 1.0 / (Mp.Z + Mm.Z), iy = 1.0 / (Mp.Y + Mm.Y);
-                wEh = R(Mp.Z * iz); wEe = R(A.c0 * iz);
-                wHe = R(Mp.Y * iy); wHh = R(A.c0 * iy);
-            } else {
-                wEh = R(0.5); wEe = R(0.5 * A.c0); wHe = R(0.5); wHh = R(0.5 * A.c0);
+        }
+
+        // ---- 3. face terms dA * F of this thread's face point(s) ----
+#pragma unroll
+        for (int t = 0; t < FPT; ++t) {
+            const int fp = tid + t * NT;
+            if (fp < EPW * NF) {
+                const int efs = fp / NF, r = fp - efs * NF;
+                const int f = r / N2, pq = r - f * N2, p = pq % N1, q = pq / N1;
+                const R* mine = &sC[fp][3];
+                R jE[3], jH[3];
+                if (cur.mirror[t]) {  // PEC mirror: [[E]] = -2 E-, [[H]] = 0 (dg_kernels.cu:244-251)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) { jE[c] = R(-2) * mine[c]; jH[c] = R(0); }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        jE[c] = cur.up[t][c] - mine[c];
+                        jH[c] = cur.up[t][c + 3] - mine[c + 3];
+                    }
+                }
+                // unit normal and area element (mesh.cpp:1032-1038)
+                const R Nx = sC[fp][0], Ny = sC[fp][1], Nz = sC[fp][2];
+                R Ln, il;
+                norm_and_inverse(Nx * Nx + Ny * Ny + Nz * Nz, Ln, il);
+                const R nx = Nx * il, ny = Ny * il, nz = Nz * il;
+                const R dA = Ln * wpq[t];
+                // central / upwind flux (dg_kernels.cu:265-294), impedance weighted with media
+                const R nxEx = ny * jE[2] - nz * jE[1], nxEy = nz * jE[0] - nx * jE[2], nxEz = nx * jE[1] - ny * jE[0];
+                const R nxHx = ny * jH[2] - nz * jH[1], nxHy = nz * jH[0] - nx * jH[2], nxHz = nx * jH[1] - ny * jH[0];
+                const R nnEx = ny * nxEz - nz * nxEy, nnEy = nz * nxEx - nx * nxEz, nnEz = nx * nxEy - ny * nxEx;
+                const R nnHx = ny * nxHz - nz * nxHy, nnHy = nz * nxHx - nx * nxHz, nnHz = nx * nxHy - ny * nxHx;
+                // F_E = wEh nx[[H]] - wEe nxnx[[E]];  F_H = -wHe nx[[E]] - wHh nxnx[[H]]
+                const R wEh = fw[t][0], wEe = fw[t][1], wHe = fw[t][2], wHh = fw[t][3];
+                R* out = sF[fp];
+                out[0] = dA * (wEh * nxHx - wEe * nnEx);
+                out[1] = dA * (wEh * nxHy - wEe * nnEy);
+                out[2] = dA * (wEh * nxHz - wEe * nnEz);
+                out[3] = dA * (-wHe * nxEx - wHh * nnHx);
+                out[4] = dA * (-wHe * nxEy - wHh * nnHy);
+                out[5] = dA * (-wHe * nxEz - wHh * nnHz);
             }
-            R* out = sF[efs][r];
-            out[0] = dA * (wEh * nxHx - wEe * nnEx);
-            out[1] = dA * (wEh * nxHy - wEe * nnEy);
-            out[2] = dA * (wEh * nxHz - wEe * nnEz);
-            out[3] = dA * (-wHe * nxEx - wHh * nnHx);
-            out[4] = dA * (-wHe * nxEy - wHh * nnHy);
-            out[5] = dA * (-wHe * nxEz - wHh * nnHz);
         }
-    }
+        __syncthreads();  // face terms complete
 
-    // ---- volume: each field's reference derivatives (local_grad3,
-    // dg_kernels.cu:101-124) folded straight into J * curl ----
-    R cE[3] = {R(0), R(0), R(0)}, cH[3] = {R(0), R(0), R(0)};
+        // ---- 4. lift: one face term per direction (the zero row off the faces).
+        // A node meets at most one face per direction; the sum (s + r) + t equals
+        // the reference's face-order gather 0..5 bit for bit (its first two
+        // terms commute exactly and a t-face always comes last) ----
+        R LE[3], LH[3];
+        {
+            const int zero = EPW * NF, base = es * NF;
+            const int sj = j == 0 ? base + 0 * N2 + i + N1 * k : (j == N1 - 1 ? base + 2 * N2 + i + N1 * k : zero);
+            const int si = i == N1 - 1 ? base + 1 * N2 + j + N1 * k : (i == 0 ? base + 3 * N2 + j + N1 * k : zero);
+            const int sk = k == 0 ? base + 4 * N2 + i + N1 * j : (k == N1 - 1 ? base + 5 * N2 + i + N1 * j : zero);
 #pragma unroll
-    for (int c = 0; c < 6; ++c) {
-        R ar = R(0), as = R(0), at = R(0);
-#pragma unroll
-        for (int q = 0; q < N1; ++q) {
-            ar += Di[q] * sU[es][c][q + N1 * j + N2 * k];
-            as += Dj[q] * sU[es][c][i + N1 * q + N2 * k];
-            at += Dk[q] * sU[es][c][i + N1 * j + N2 * q];
+            for (int c = 0; c < 3; ++c) {
+                LE[c] = (sF[sj][c] + sF[si][c]) + sF[sk][c];
+                LH[c] = (sF[sj][c + 3] + sF[si][c + 3]) + sF[sk][c + 3];
+            }
         }
-        const R dxc = ar * Crx + as * Csx + at * Ctx;  // J d/dx (dg_kernels.cu:140-149, W*J*J^-T = W*C)
-        const R dyc = ar * Cry + as * Csy + at * Cty;
-        const R dzc = ar * Crz + as * Csz + at * Ctz;
-        R* cu = c < 3 ? cE : cH;
-        switch (c % 3) {  // curl u = (dy u_z - dz u_y, dz u_x - dx u_z, dx u_y - dy u_x)
-            case 0: cu[1] += dzc; cu[2] -= dyc; break;
-            case 1: cu[0] -= dzc; cu[2] += dxc; break;
-            default: cu[0] += dyc; cu[1] -= dxc; break;
-        }
-        // pin this field's arithmetic here: one field's LDS reads live at a time
-        pin(cE[0]); pin(cE[1]); pin(cE[2]); pin(cH[0]); pin(cH[1]); pin(cH[2]);
-    }
-    __syncthreads();  // face terms complete
 
-    // ---- lift: this node's face terms in face order 0..5 (deterministic) ----
-    R LE[3] = {R(0), R(0), R(0)}, LH[3] = {R(0), R(0), R(0)};
-    auto add = [&](int f, int pq) {
-        const R* s = sF[es][f * N2 + pq];
-        LE[0] += s[0]; LE[1] += s[1]; LE[2] += s[2];
-        LH[0] += s[3]; LH[1] += s[4]; LH[2] += s[5];
-    };
-    if (j == 0) add(0, i + N1 * k);
-    if (i == N1 - 1) add(1, j + N1 * k);
-    if (j == N1 - 1) add(2, i + N1 * k);
-    if (i == 0) add(3, j + N1 * k);
-    if (k == 0) add(4, i + N1 * j);
-    if (k == N1 - 1) add(5, i + N1 * j);
-
-    // ---- inverse mass + LSRK45 update (dg_kernels.cu:353-408) ----
-    const ElemMat M = A.mat[A.elem_base + ee];
-    const R Winv = R(c_gll.iw[N1][i] * c_gll.iw[N1][j] * c_gll.iw[N1][k]);
-    const R sE = R(M.ieps) * iJ, sH = R(M.imu) * iJ;
-    if (live) {
-        const double dt = *A.dt;
+        // ---- inverse mass + LSRK45 update (dg_kernels.cu:353-408) ----
+        // Stored by every thread: the spare slots of a last, partly filled
+        // group repeat its last element with the same inputs and arithmetic,
+        // so they store the same values to the same places (no branch: a
+        // store on one path only would make the compiler's memory-counter
+        // waits conservative and stall on the stores).
+        {
+            const R sE_ = R(ER.ieps) * iJ, sH_ = R(ER.imu) * iJ;
 #pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            const R rhs = c < 3 ? sE * (cH[c] + LE[c] * Winv) : sH * (-cE[c - 3] + LH[c - 3] * Winv);
-            const KR kn = FIRST ? KR(dt) * KR(rhs) : KR(A.a) * kold[c] + KR(dt) * KR(rhs);
-            kb[c * KS + node] = kn;
-            uout[c * S + node] = R(KR(u[c]) + KR(A.b) * kn);
+            for (int c = 0; c < 6; ++c) {
+                const R rhs = c < 3 ? sE_ * (cH[c] + LE[c] * Winv) : sH_ * (-cE[c - 3] + LH[c - 3] * Winv);
+                const KR kn = FIRST ? KR(dt) * KR(rhs) : KR(A.a) * kold[c] + KR(dt) * KR(rhs);
+                bstore(kn, rK, nd * unsigned(sizeof(KR)), c * sK);
+                bstore(R(KR(sU[es][c][m]) + KR(A.b) * kn), rO, nd * unsigned(sizeof(R)), c * sS);
+            }
         }
+        cur = nxt;
+        nb1 = nb2;
+        rec1 = rec2;
     }
 }
 
@@ -316,16 +466,39 @@ constexpr int epw_of(int n1, int opt, int rsz) {
     return want < cap ? want : (cap > 0 ? cap : 1);
 }
 
+// Workgroups resident per CU for one kernel instance (both FIRST variants
+// share the resource footprint), times the CU count: the persistent grid.
+template <typename K>
+int resident_groups(K kernel, int threads) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess) return 0;
+    return (per_cu > 0 ? per_cu : 1) * cus;
+}
+
 template <int N1, int OPT, typename R, typename KR>
 hipError_t launch_one(const FusedArgs& a, hipStream_t s) {
     constexpr int EPW = epw_of(N1, OPT, sizeof(R));
-    const int nblocks = (a.nelt + EPW - 1) / EPW;
+    constexpr int NT = EPW * N1 * N1 * N1;
+    static int resident = 0;
+    if (resident <= 0) resident = resident_groups(k_fused_stage<N1, EPW, false, R, KR>, NT);
+    if (resident <= 0) return hipErrorInvalidValue;
+    // every byte offset (component block included) of a field, register or
+    // ghost slot fits in 32 bits
+    const unsigned long long span = 6ull * (unsigned long long)(a.stride > a.kstride ? a.stride : a.kstride);
+    if (span * sizeof(double) >= (1ull << 32)) return hipErrorInvalidValue;
     FusedArgs b = a;
-    b.nchunk = (nblocks + 7) / 8;
+    b.ngroups = (a.nelt + EPW - 1) / EPW;
+    const int want = b.ngroups < resident ? b.ngroups : resident;
+    b.gper = (b.ngroups + want - 1) / want;
+    const int used = (b.ngroups + b.gper - 1) / b.gper;
+    b.wchunk = (used + 7) / 8;
+    const dim3 grid(8 * b.wchunk), block(NT);
     if (a.first_stage)
-        hipLaunchKernelGGL((k_fused_stage<N1, EPW, true, R, KR>), dim3(8 * b.nchunk), dim3(EPW * N1 * N1 * N1), 0, s, b);
+        hipLaunchKernelGGL((k_fused_stage<N1, EPW, true, R, KR>), grid, block, 0, s, b);
     else
-        hipLaunchKernelGGL((k_fused_stage<N1, EPW, false, R, KR>), dim3(8 * b.nchunk), dim3(EPW * N1 * N1 * N1), 0, s, b);
+        hipLaunchKernelGGL((k_fused_stage<N1, EPW, false, R, KR>), grid, block, 0, s, b);
     return hipGetLastError();
 }
 

@@ -24,13 +24,15 @@ struct FusedArgs {
     int gper;                 // groups per workgroup (persistent loop)
     int wchunk;               // workgroups per XCD (XCD-aware remap)
     const ElemGeom* geom;     // per element, segment-local
+    const double* rec;        // per element, segment-local: 24 trilinear coefficients, 1/eps, 1/mu
     const ElemMat* mat;       // per element, global index (always present; vacuum = 1)
     const ElemMat* gmat;      // per ghost face
     const int4* fnb;          // per element face (nelt*6): neighbour addressing, see FaceNbr
     const double* dt;         // device scalar
     double a, b, c0;
     int first_stage;          // a == 0: k is not read
-    int weighted;             // impedance-weighted flux (media present)
+    const void* fw;           // per element face (nelt*6, segment-local), 4 R: flux weights
+                              // (wEh, wEe, wHe, wHh), see face_flux_weights
 };
 
 // Elements per workgroup options (index 0/1) for a given n1.
@@ -66,6 +68,25 @@ __host__ __device__ inline void face_ijk(int f, int n1, int p, int q, int& i, in
 // and w names the plus side's medium: >= 0 element index, < 0 ghost face -(w+1).
 int make_face_nbr(const int* fcode, int nelt, int n1, long long node_base, long long ghost_base, int elem_base,
                   int4* out);
+
+// Flux weights of one face for the minus/plus media (Z = sqrt(mu/eps), Y = 1/Z):
+//   F_E = wEh n x [[H]] - wEe n x n x [[E]],  F_H = -wHe n x [[E]] - wHh n x n x [[H]]
+// with wEh = Z+/(Z+ + Z-), wEe = c0/(Z+ + Z-), wHe = Y+/(Y+ + Y-), wHh = c0/(Y+ + Y-);
+// without media the reference's 1/2, c0/2 (dg_kernels.cu:265-294).
+inline void face_flux_weights(const ElemMat& mm, const ElemMat& mp, double c0, bool media, double w[4]) {
+    if (media) {
+        const double iz = 1.0 / (mp.Z + mm.Z), iy = 1.0 / (mp.Y + mm.Y);
+        w[0] = mp.Z * iz;
+        w[1] = c0 * iz;
+        w[2] = mp.Y * iy;
+        w[3] = c0 * iy;
+    } else {
+        w[0] = 0.5;
+        w[1] = 0.5 * c0;
+        w[2] = 0.5;
+        w[3] = 0.5 * c0;
+    }
+}
 
 // fcode encoding: >= 0 : nbr_elem * 64 + nbr_face * 8 + orient; -1 : mirror; <= -2 : ghost face -(c+2)
 __host__ __device__ inline int fcode_pack(int e, int f, int o) { return e * 64 + f * 8 + o; }

@@ -180,6 +180,13 @@ int Solver::init(const Mesh* const* meshes, int count, double c0_, const nw_opts
         }
         if ((st = upload_vec(&d_mat, mat))) return st;
         if ((st = upload_vec(&d_geom, geom))) return st;
+        std::vector<double> rec(size_t(nelt) * 26);
+        for (int e = 0; e < nelt; ++e) {
+            std::copy(&geom[e].c[0][0], &geom[e].c[0][0] + 24, &rec[size_t(e) * 26]);
+            rec[size_t(e) * 26 + 24] = mat[e].ieps;
+            rec[size_t(e) * 26 + 25] = mat[e].imu;
+        }
+        if ((st = upload_vec(&d_rec, rec))) return st;
     }
 
     // halo stand-in metadata (dg_solver.cpp:131-170 with one neighbour: self)
@@ -221,6 +228,26 @@ int Solver::init(const Mesh* const* meshes, int count, double c0_, const nw_opts
             }
         }
         if ((st = upload_vec(&d_fnb, fnb))) return st;
+        // flux weights per element face from the two media (the plus side of a
+        // mirror face is the element itself; of a cut face, its ghost medium)
+        std::vector<double> w64(size_t(nelt) * 24);
+        for (int r = 0; r < count; ++r) {
+            const Instance& I = inst[r];
+            const Mesh& m = *meshes[r];
+            for (int e = 0; e < I.nelt; ++e)
+                for (int f = 0; f < 6; ++f) {
+                    const int code = m.fcode[size_t(e) * 6 + f];
+                    const ElemMat& mm = m.mat[e];
+                    const ElemMat& mp = code >= 0 ? m.mat[code >> 6] : (code == -1 ? mm : m.gmat[-(code + 2)]);
+                    face_flux_weights(mm, mp, c0, media, &w64[(size_t(I.elem_base + e) * 6 + f) * 4]);
+                }
+        }
+        if (rsz == 8) {
+            if ((st = upload_vec(reinterpret_cast<double**>(&d_fw), w64))) return st;
+        } else {
+            std::vector<float> w32(w64.begin(), w64.end());
+            if ((st = upload_vec(reinterpret_cast<float**>(&d_fw), w32))) return st;
+        }
         for (size_t x = 1; x < seg.size(); ++x) {
             hipStream_t s2;
             NW_TRY_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
@@ -350,6 +377,7 @@ int Solver::launch_segment(const Segment& sg, int s, int parity, int option, hip
     a.nelt = sg.nelt;
     a.elem_base = sg.elem_base;
     a.geom = d_geom + sg.elem_base;
+    a.rec = d_rec + size_t(sg.elem_base) * 26;
     a.mat = d_mat;
     a.gmat = d_gmat;
     a.fnb = d_fnb + size_t(sg.elem_base) * 6;
@@ -358,7 +386,7 @@ int Solver::launch_segment(const Segment& sg, int s, int parity, int option, hip
     a.b = kRkB[s];
     a.c0 = c0;
     a.first_stage = (s == 0);
-    a.weighted = media ? 1 : 0;
+    a.fw = static_cast<const char*>(d_fw) + size_t(sg.elem_base) * 24 * (prec == kFP64 ? 8 : 4);
     NW_TRY_HIP(launch_fused(sg.n1, prec, option, a, st));
     return NW_OK;
 }

@@ -52,6 +52,8 @@ struct Solver {
     void* d_u[2] = {nullptr, nullptr};
     void* d_k = nullptr;
     int4* d_fnb = nullptr;
+    void* d_fw = nullptr;  // flux weights per element face, 4 values of the field type
+    double* d_rec = nullptr;  // per element: 24 trilinear coefficients, 1/eps, 1/mu
     // plain pipeline (reference residency, single mesh)
     double *d_state = nullptr, *d_kk = nullptr, *d_rhs = nullptr, *d_D = nullptr, *d_w3 = nullptr;
     double* d_met[13] = {};  // x y z jac rx sx tx ry sy ty rz sz tz
