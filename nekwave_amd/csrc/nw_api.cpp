@@ -379,12 +379,12 @@ NW_API int nw_launch_lsrk45_update(double* st, double* k, const double* rhs, dou
 }
 NW_API int nw_launch_pack_halo(const double* st, double* buf, const int* vol, const int* off, const int* sizes,
                                int nh, int64_t S, void* stream) {
-    NW_TRY_HIP(nw::launch_pack<double>(st, buf, vol, off, sizes, nh, S, hs(stream)));
+    NW_TRY_HIP(nw::launch_pack<double>(st, buf, vol, off, sizes, nh, S, 1, hs(stream)));
     return NW_OK;
 }
 NW_API int nw_launch_unpack_halo(const double* buf, double* st, const int* off, const int* sizes, int nh,
                                  int64_t npts, int64_t S, void* stream) {
-    NW_TRY_HIP(nw::launch_unpack<double>(buf, st, off, sizes, nh, npts, S, hs(stream)));
+    NW_TRY_HIP(nw::launch_unpack<double>(buf, st, off, sizes, nh, npts, S, 1, hs(stream)));
     return NW_OK;
 }
 

@@ -102,20 +102,68 @@ __device__ __forceinline__ float bload(float*, __amdgpu_buffer_rsrc_t r, unsigne
 __device__ __forceinline__ double bload(double*, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
 }
-__device__ __forceinline__ void bstore(float v, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vo, so, 0);
-}
-__device__ __forceinline__ void bstore(double v, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, vo,
-                                          so, 0);
-}
 template <typename T>
 __device__ __forceinline__ T bld(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
     return bload(static_cast<T*>(nullptr), r, vo, so);
 }
 __device__ __forceinline__ int4 bld_int4(__amdgpu_buffer_rsrc_t r, unsigned vo) {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, vo, 0, 0);
-    return make_int4(int(v[0]), int(v[1]), int(v[2]), int(v[3]));
+    typedef __attribute__((ext_vector_type(4))) int i32x4;
+    const i32x4 v = __builtin_bit_cast(i32x4, __builtin_amdgcn_raw_buffer_load_b128(r, vo, 0, 0));
+    return make_int4(v[0], v[1], v[2], v[3]);
+}
+// The six field values of one node (node-major state: 6 consecutive values
+// per node).  Accesses stay naturally aligned: FP32 nodes sit 24 bytes apart,
+// so three 8-byte accesses; FP64 nodes 48 bytes apart, so three 16-byte ones.
+typedef __attribute__((ext_vector_type(2))) unsigned u32x2;
+typedef __attribute__((ext_vector_type(4))) unsigned u32x4;
+typedef __attribute__((ext_vector_type(2))) float f32x2;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+typedef __attribute__((ext_vector_type(2))) double f64x2;
+// (Each 8-byte word is reinterpreted whole as two floats: extracting single
+// 32-bit lanes of the loaded vector before the reinterpretation let the
+// compiler narrow the access to its first lane.)
+__device__ __forceinline__ void ld6(__amdgpu_buffer_rsrc_t r, unsigned vo, float (&o)[6]) {
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        const f32x2 a = __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(r, vo + 8u * x, 0, 0));
+        o[2 * x] = a[0];
+        o[2 * x + 1] = a[1];
+    }
+}
+__device__ __forceinline__ void ld6(__amdgpu_buffer_rsrc_t r, unsigned vo, double (&o)[6]) {
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        const f64x2 a = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(r, vo + 16u * x, 0, 0));
+        o[2 * x] = a[0];
+        o[2 * x + 1] = a[1];
+    }
+}
+__device__ __forceinline__ void ld4(__amdgpu_buffer_rsrc_t r, unsigned vo, float (&o)[4]) {
+    const f32x4 a = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, vo, 0, 0));
+#pragma unroll
+    for (int c = 0; c < 4; ++c) o[c] = a[c];
+}
+__device__ __forceinline__ void ld4(__amdgpu_buffer_rsrc_t r, unsigned vo, double (&o)[4]) {
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+        const f64x2 a = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(r, vo + 16u * x, 0, 0));
+        o[2 * x] = a[0];
+        o[2 * x + 1] = a[1];
+    }
+}
+__device__ __forceinline__ void st6(__amdgpu_buffer_rsrc_t r, unsigned vo, const float (&v)[6]) {
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        const f32x2 a = {v[2 * x], v[2 * x + 1]};
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, a), r, vo + 8u * x, 0, 0);
+    }
+}
+__device__ __forceinline__ void st6(__amdgpu_buffer_rsrc_t r, unsigned vo, const double (&v)[6]) {
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        const f64x2 a = {v[2 * x], v[2 * x + 1]};
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, a), r, vo + 16u * x, 0, 0);
+    }
 }
 
 // 1/J of the FP64 Jacobian: in FP32 the hardware reciprocal (1 ulp) of the
@@ -179,8 +227,7 @@ void k_fused_stage(FusedArgs A) {
     const __amdgpu_buffer_rsrc_t rU = make_rsrc(A.uin), rO = make_rsrc(A.uout), rK = make_rsrc(A.kbuf);
     const __amdgpu_buffer_rsrc_t rW = make_rsrc(A.fw), rN = make_rsrc(A.fnb);
     const __amdgpu_buffer_rsrc_t rR = make_rsrc(A.rec);
-    const unsigned sS = unsigned(A.stride) * unsigned(sizeof(R));    // component stride of u, bytes
-    const unsigned sK = unsigned(A.kstride) * unsigned(sizeof(KR));  // component stride of k, bytes
+    constexpr unsigned UB = 6 * sizeof(R), KB = 6 * sizeof(KR);  // bytes per node of u and of k
 
     // loop-invariant: node coordinates, inverse mass weight
     const double zr = c_gll.z[N1][i], zs = c_gll.z[N1][j], zt = c_gll.z[N1][k];
@@ -243,8 +290,7 @@ void k_fused_stage(FusedArgs A) {
     };
     auto fetch = [&](int g, const Nbr& N, Pre& P) {
         const unsigned nd = node_of(g);
-#pragma unroll
-        for (int c = 0; c < 6; ++c) P.u[c] = bld<R>(rU, nd * unsigned(sizeof(R)), c * sS);
+        ld6(rU, nd * UB, P.u);
         // issued by every thread (threads without a face point read their own
         // node): the same number of loads on every path keeps the compiler's
         // memory-counter waits exact instead of conservative
@@ -256,8 +302,7 @@ void k_fused_stage(FusedArgs A) {
             const int4 b = N.b[t];
             P.mirror[t] = b.x < 0;
             const unsigned idx = (b.x >= 0 && fp < EPW * NF) ? unsigned(b.x + p * b.y + q * b.z) : nd;
-#pragma unroll
-            for (int c = 0; c < 6; ++c) P.up[t][c] = bld<R>(rU, idx * unsigned(sizeof(R)), c * sS);
+            ld6(rU, idx * UB, P.up[t]);
         }
     };
 
@@ -285,17 +330,15 @@ void k_fused_stage(FusedArgs A) {
         // reads of this group needed late (LSRK registers, flux weights) go
         // before the prefetch: waiting on them does not wait on it
         const unsigned nd = node_of(g);
-        KR kold[6];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) kold[c] = FIRST ? KR(0) : bld<KR>(rK, nd * unsigned(sizeof(KR)), c * sK);
+        KR kold[6] = {KR(0), KR(0), KR(0), KR(0), KR(0), KR(0)};
+        if (!FIRST) ld6(rK, nd * KB, kold);
         R fw[FPT][4];
 #pragma unroll
         for (int t = 0; t < FPT; ++t) {
             const int fp = tid + t * NT;
             const int efs = min(fp / NF, EPW - 1), f = (fp - (fp / NF) * NF) / N2;
             const unsigned face = unsigned(clamp_e(g, efs) * 6 + f);
-#pragma unroll
-            for (int c = 0; c < 4; ++c) fw[t][c] = bld<R>(rW, (face * 4u + unsigned(c)) * unsigned(sizeof(R)), 0);
+            ld4(rW, face * unsigned(4 * sizeof(R)), fw[t]);
         }
         Pre nxt;
         fetch(min(g + 1, g1 - 1), nb1, nxt);  // in flight while this group computes
@@ -351,23 +394,26 @@ void k_fused_stage(FusedArgs A) {
             const R Cx = d == 0 ? Crx : (d == 1 ? Csx : Ctx);
             const R Cy = d == 0 ? Cry : (d == 1 ? Csy : Cty);
             const R Cz = d == 0 ? Crz : (d == 1 ? Csz : Ctz);
+            // E then H: three fields' contractions as independent chains, interleaved
 #pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                R a = R(0);
+            for (int h = 0; h < 2; ++h) {
+                R a[3] = {R(0), R(0), R(0)};
 #pragma unroll
-                for (int q = 0; q < N1; ++q)
-                    a += Dr[q] * sU[es][c][d == 0 ? q + N1 * j + N2 * k : (d == 1 ? i + N1 * q + N2 * k : i + N1 * j + N2 * q)];
-                // J d/dx_e of field c gets a * C_d,e; curl u = (dy u_z - dz u_y, dz u_x - dx u_z, dx u_y - dy u_x)
-                R* cv = c < 3 ? cE : cH;
-                switch (c % 3) {
-                    case 0: cv[1] += a * Cz; cv[2] -= a * Cy; break;
-                    case 1: cv[0] -= a * Cz; cv[2] += a * Cx; break;
-                    default: cv[0] += a * Cy; cv[1] -= a * Cx; break;
+                for (int q = 0; q < N1; ++q) {
+                    const int at = d == 0 ? q + N1 * j + N2 * k : (d == 1 ? i + N1 * q + N2 * k : i + N1 * j + N2 * q);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) a[c] += Dr[q] * sU[es][3 * h + c][at];
                 }
-                // keep each contraction here: sinking it into the update below
-                // would hold the fields' LDS values in registers
-                pin(cE[0]); pin(cE[1]); pin(cE[2]); pin(cH[0]); pin(cH[1]); pin(cH[2]);
+                // J d/dx_e of component c gets a * C_d,e;
+                // curl u = (dy u_z - dz u_y, dz u_x - dx u_z, dx u_y - dy u_x)
+                R* cv = h == 0 ? cE : cH;
+                cv[1] += a[0] * Cz; cv[2] -= a[0] * Cy;
+                cv[0] -= a[1] * Cz; cv[2] += a[1] * Cx;
+                cv[0] += a[2] * Cy; cv[1] -= a[2] * Cx;
             }
+            // keep each direction's contraction here: sinking it into the update
+            // below would hold the fields' LDS values in registers
+            pin(cE[0]); pin(cE[1]); pin(cE[2]); pin(cH[0]); pin(cH[1]); pin(cH[2]);
         }
 
         // ---- 3. face terms dA * F of this thread's face point(s) ----
@@ -438,13 +484,16 @@ void k_fused_stage(FusedArgs A) {
         // waits conservative and stall on the stores).
         {
             const R sE_ = R(ER.ieps) * iJ, sH_ = R(ER.imu) * iJ;
+            KR kn[6];
+            R un[6];
 #pragma unroll
             for (int c = 0; c < 6; ++c) {
                 const R rhs = c < 3 ? sE_ * (cH[c] + LE[c] * Winv) : sH_ * (-cE[c - 3] + LH[c - 3] * Winv);
-                const KR kn = FIRST ? KR(dt) * KR(rhs) : KR(A.a) * kold[c] + KR(dt) * KR(rhs);
-                bstore(kn, rK, nd * unsigned(sizeof(KR)), c * sK);
-                bstore(R(KR(sU[es][c][m]) + KR(A.b) * kn), rO, nd * unsigned(sizeof(R)), c * sS);
+                kn[c] = FIRST ? KR(dt) * KR(rhs) : KR(A.a) * kold[c] + KR(dt) * KR(rhs);
+                un[c] = R(KR(sU[es][c][m]) + KR(A.b) * kn[c]);
             }
+            st6(rK, nd * KB, kn);
+            st6(rO, nd * UB, un);
         }
         cur = nxt;
         nb1 = nb2;

@@ -205,22 +205,22 @@ __global__ void k_lsrk(double* __restrict__ st, double* __restrict__ k, const do
 
 template <typename R>
 __global__ void k_pack(const R* __restrict__ st, R* __restrict__ buf, const int* __restrict__ vol,
-                       const int* __restrict__ off, const int* __restrict__ sizes, int nh, long long S) {
+                       const int* __restrict__ off, const int* __restrict__ sizes, int nh, long long cs, long long ns) {
     for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < nh; x += gridDim.x * blockDim.x) {
         const long long v = vol[x];
         const int base = off[x], M = sizes[x];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) buf[base + c * M] = st[c * S + v];
+        for (int c = 0; c < 6; ++c) buf[base + c * M] = st[c * cs + v * ns];
     }
 }
 
 template <typename R>
 __global__ void k_unpack(const R* __restrict__ buf, R* __restrict__ st, const int* __restrict__ off,
-                         const int* __restrict__ sizes, int nh, long long npts, long long S) {
+                         const int* __restrict__ sizes, int nh, long long npts, long long cs, long long ns) {
     for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < nh; x += gridDim.x * blockDim.x) {
         const int base = off[x], M = sizes[x];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) st[c * S + npts + x] = buf[base + c * M];
+        for (int c = 0; c < 6; ++c) st[c * cs + (npts + x) * ns] = buf[base + c * M];
     }
 }
 
@@ -316,7 +316,8 @@ __global__ void k_face_data(const int* __restrict__ fcode, const int* __restrict
 // diagnostics: energy (FP64), max |E|, max |H|; two-pass deterministic reduce
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ void k_diag_partial(const R* __restrict__ u, long long S, long long npts, int n1, const double* __restrict__ jac,
+__global__ void k_diag_partial(const R* __restrict__ u, long long cs, long long ns, long long npts, int n1,
+                               const double* __restrict__ jac,
                                const ElemGeom* __restrict__ geom, const ElemMat* __restrict__ mat,
                                double* __restrict__ part) {
     __shared__ double sh[3][kBlock];
@@ -346,8 +347,9 @@ __global__ void k_diag_partial(const R* __restrict__ u, long long S, long long n
             J = xr * (ys * zt - yt * zs) - xs * (yr * zt - yt * zr) + xt * (yr * zs - ys * zr);
         }
         const double dV = J * c_gll.w[n1][i] * c_gll.w[n1][j] * c_gll.w[n1][k];
-        const double ex = u[p], ey = u[S + p], ez = u[2 * S + p];
-        const double hx = u[3 * S + p], hy = u[4 * S + p], hz = u[5 * S + p];
+        const R* up = u + p * ns;
+        const double ex = up[0], ey = up[cs], ez = up[2 * cs];
+        const double hx = up[3 * cs], hy = up[4 * cs], hz = up[5 * cs];
         const double e2 = ex * ex + ey * ey + ez * ez, h2 = hx * hx + hy * hy + hz * hz;
         const double ep = mat ? 1.0 / mat[e].ieps : 1.0, mu = mat ? 1.0 / mat[e].imu : 1.0;
         en += 0.5 * (ep * e2 + mu * h2) * dV;
@@ -383,12 +385,12 @@ __global__ void k_diag_final(const double* __restrict__ part, int n, double* __r
 // conversions and plain copy (HBM calibration)
 // ---------------------------------------------------------------------------
 template <typename D, typename S>
-__global__ void k_convert(D* __restrict__ dst, long long dstride, const S* __restrict__ src, long long sstride,
-                          long long n, int ncomp) {
+__global__ void k_convert(D* __restrict__ dst, long long dcs, long long dns, const S* __restrict__ src, long long scs,
+                          long long sns, long long n, int ncomp) {
     const long long tot = n * ncomp;
     for (long long x = blockIdx.x * (long long)blockDim.x + threadIdx.x; x < tot; x += (long long)gridDim.x * blockDim.x) {
         const long long c = x / n, p = x - c * n;
-        dst[c * dstride + p] = static_cast<D>(src[c * sstride + p]);
+        dst[c * dcs + p * dns] = static_cast<D>(src[c * scs + p * sns]);
     }
 }
 
@@ -467,23 +469,27 @@ hipError_t launch_lsrk(double* st, double* k, const double* rhs, double a, doubl
 }
 
 template <typename R>
-hipError_t launch_pack(const R* st, R* buf, const int* vol, const int* off, const int* sizes, int nh, long long S,
-                       hipStream_t s) {
+hipError_t launch_pack(const R* st, R* buf, const int* vol, const int* off, const int* sizes, int nh, long long cs,
+                       long long ns, hipStream_t s) {
     if (nh <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_pack<R>, dim3(grid_for(nh)), dim3(kBlock), 0, s, st, buf, vol, off, sizes, nh, S);
+    hipLaunchKernelGGL(k_pack<R>, dim3(grid_for(nh)), dim3(kBlock), 0, s, st, buf, vol, off, sizes, nh, cs, ns);
     return hipGetLastError();
 }
 template <typename R>
-hipError_t launch_unpack(const R* buf, R* st, const int* off, const int* sizes, int nh, long long npts, long long S,
-                         hipStream_t s) {
+hipError_t launch_unpack(const R* buf, R* st, const int* off, const int* sizes, int nh, long long npts, long long cs,
+                         long long ns, hipStream_t s) {
     if (nh <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_unpack<R>, dim3(grid_for(nh)), dim3(kBlock), 0, s, buf, st, off, sizes, nh, npts, S);
+    hipLaunchKernelGGL(k_unpack<R>, dim3(grid_for(nh)), dim3(kBlock), 0, s, buf, st, off, sizes, nh, npts, cs, ns);
     return hipGetLastError();
 }
-template hipError_t launch_pack<double>(const double*, double*, const int*, const int*, const int*, int, long long, hipStream_t);
-template hipError_t launch_pack<float>(const float*, float*, const int*, const int*, const int*, int, long long, hipStream_t);
-template hipError_t launch_unpack<double>(const double*, double*, const int*, const int*, int, long long, long long, hipStream_t);
-template hipError_t launch_unpack<float>(const float*, float*, const int*, const int*, int, long long, long long, hipStream_t);
+template hipError_t launch_pack<double>(const double*, double*, const int*, const int*, const int*, int, long long,
+                                        long long, hipStream_t);
+template hipError_t launch_pack<float>(const float*, float*, const int*, const int*, const int*, int, long long, long long,
+                                       hipStream_t);
+template hipError_t launch_unpack<double>(const double*, double*, const int*, const int*, int, long long, long long,
+                                          long long, hipStream_t);
+template hipError_t launch_unpack<float>(const float*, float*, const int*, const int*, int, long long, long long,
+                                         long long, hipStream_t);
 
 hipError_t launch_metrics(const double* corners, int n1, long long npts, double* const* out13, hipStream_t s) {
     if (npts <= 0) return hipSuccess;
@@ -506,30 +512,34 @@ hipError_t launch_face_data(const int* fcode, const int* fbc, int n1, long long 
 int diag_blocks() { return 1024; }
 
 template <typename R>
-hipError_t launch_diag(const R* u, long long S, long long npts, int n1, const double* jac, const ElemGeom* geom,
-                       const ElemMat* mat, double* part, double* out, hipStream_t s) {
+hipError_t launch_diag(const R* u, long long cs, long long ns, long long npts, int n1, const double* jac,
+                       const ElemGeom* geom, const ElemMat* mat, double* part, double* out, hipStream_t s) {
     const int nb = diag_blocks();
-    hipLaunchKernelGGL(k_diag_partial<R>, dim3(nb), dim3(kBlock), 0, s, u, S, npts, n1, jac, geom, mat, part);
+    hipLaunchKernelGGL(k_diag_partial<R>, dim3(nb), dim3(kBlock), 0, s, u, cs, ns, npts, n1, jac, geom, mat, part);
     hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s, part, nb, out);
     return hipGetLastError();
 }
-template hipError_t launch_diag<double>(const double*, long long, long long, int, const double*, const ElemGeom*,
-                                        const ElemMat*, double*, double*, hipStream_t);
-template hipError_t launch_diag<float>(const float*, long long, long long, int, const double*, const ElemGeom*,
+template hipError_t launch_diag<double>(const double*, long long, long long, long long, int, const double*,
+                                        const ElemGeom*, const ElemMat*, double*, double*, hipStream_t);
+template hipError_t launch_diag<float>(const float*, long long, long long, long long, int, const double*, const ElemGeom*,
                                        const ElemMat*, double*, double*, hipStream_t);
 
 template <typename D, typename S>
-hipError_t launch_convert(D* dst, long long dstride, const S* src, long long sstride, long long n, int ncomp,
-                          hipStream_t s) {
+hipError_t launch_convert(D* dst, long long dcs, long long dns, const S* src, long long scs, long long sns, long long n,
+                          int ncomp, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_convert<D, S>), dim3(grid_for(n * ncomp)), dim3(kBlock), 0, s, dst, dstride, src, sstride, n,
+    hipLaunchKernelGGL((k_convert<D, S>), dim3(grid_for(n * ncomp)), dim3(kBlock), 0, s, dst, dcs, dns, src, scs, sns, n,
                        ncomp);
     return hipGetLastError();
 }
-template hipError_t launch_convert<float, double>(float*, long long, const double*, long long, long long, int, hipStream_t);
-template hipError_t launch_convert<double, float>(double*, long long, const float*, long long, long long, int, hipStream_t);
-template hipError_t launch_convert<double, double>(double*, long long, const double*, long long, long long, int, hipStream_t);
-template hipError_t launch_convert<float, float>(float*, long long, const float*, long long, long long, int, hipStream_t);
+#define NW_CONVERT(D, S)                                                                                        \
+    template hipError_t launch_convert<D, S>(D*, long long, long long, const S*, long long, long long, long long, int, \
+                                             hipStream_t);
+NW_CONVERT(float, double)
+NW_CONVERT(double, float)
+NW_CONVERT(double, double)
+NW_CONVERT(float, float)
+#undef NW_CONVERT
 
 hipError_t launch_copy(const void* src, void* dst, long long bytes, hipStream_t s) {
     const long long n4 = bytes / 16;

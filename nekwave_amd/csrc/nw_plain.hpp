@@ -30,23 +30,25 @@ hipError_t launch_inv_mass(double* rhs, const double* jac, const double* w3, int
                            const double* eps, const double* mu, hipStream_t s);
 hipError_t launch_lsrk(double* st, double* k, const double* rhs, double a, double b, const double* dtp, double dt,
                        long long npts, long long S, hipStream_t s);
+// State layouts: component c of node p at c * cs + p * ns (the reference's
+// blocks [Ex|..|Hz]: cs = stride, ns = 1; node-major: cs = 1, ns = 6).
 template <typename R>
-hipError_t launch_pack(const R* st, R* buf, const int* vol, const int* off, const int* sizes, int nh, long long S,
-                       hipStream_t s);
+hipError_t launch_pack(const R* st, R* buf, const int* vol, const int* off, const int* sizes, int nh, long long cs,
+                       long long ns, hipStream_t s);
 template <typename R>
-hipError_t launch_unpack(const R* buf, R* st, const int* off, const int* sizes, int nh, long long npts, long long S,
-                         hipStream_t s);
+hipError_t launch_unpack(const R* buf, R* st, const int* off, const int* sizes, int nh, long long npts, long long cs,
+                         long long ns, hipStream_t s);
 hipError_t launch_metrics(const double* corners, int n1, long long npts, double* const* out13, hipStream_t s);
 hipError_t launch_face_data(const int* fcode, const int* fbc, int n1, long long tfp, long long npts,
                             const double* const* met10, int* vM, int* vP, int* bc, double* nx, double* ny, double* nz,
                             double* dA, hipStream_t s);
 int diag_blocks();
 template <typename R>
-hipError_t launch_diag(const R* u, long long S, long long npts, int n1, const double* jac, const ElemGeom* geom,
-                       const ElemMat* mat, double* part, double* out, hipStream_t s);
+hipError_t launch_diag(const R* u, long long cs, long long ns, long long npts, int n1, const double* jac,
+                       const ElemGeom* geom, const ElemMat* mat, double* part, double* out, hipStream_t s);
 template <typename D, typename S>
-hipError_t launch_convert(D* dst, long long dstride, const S* src, long long sstride, long long n, int ncomp,
-                          hipStream_t s);
+hipError_t launch_convert(D* dst, long long dcs, long long dns, const S* src, long long scs, long long sns, long long n,
+                          int ncomp, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, long long bytes, hipStream_t s);
 hipError_t launch_fill(double* p, long long n, double v, hipStream_t s);
 

@@ -323,13 +323,16 @@ int Solver::upload(int ix, const double* host, size_t n) {
     const long long P = I.npts, S = stride;
     if (pipe == NW_PIPE_PLAIN) {
         NW_TRY_HIP(hipMemcpy2DAsync(d_state, S * 8, host, P * 8, P * 8, 6, hipMemcpyHostToDevice, stream));
-    } else if (prec == kFP64) {
-        double* dst = static_cast<double*>(d_u[cur]) + I.node_base;
-        NW_TRY_HIP(hipMemcpy2DAsync(dst, S * 8, host, P * 8, P * 8, 6, hipMemcpyHostToDevice, stream));
     } else {
+        // host blocks [Ex|..|Hz] -> node-major device layout (6 values per node)
         NW_TRY_HIP(hipMemcpyAsync(d_stage, host, 8 * 6 * P, hipMemcpyHostToDevice, stream));
-        float* dst = static_cast<float*>(d_u[cur]) + I.node_base;
-        NW_TRY_HIP((launch_convert<float, double>(dst, S, d_stage, P, P, 6, stream)));
+        if (prec == kFP64) {
+            double* dst = static_cast<double*>(d_u[cur]) + 6 * I.node_base;
+            NW_TRY_HIP((launch_convert<double, double>(dst, 1, 6, d_stage, P, 1, P, 6, stream)));
+        } else {
+            float* dst = static_cast<float*>(d_u[cur]) + 6 * I.node_base;
+            NW_TRY_HIP((launch_convert<float, double>(dst, 1, 6, d_stage, P, 1, P, 6, stream)));
+        }
     }
     NW_TRY_HIP(hipStreamSynchronize(stream));
     uploaded = true;
@@ -345,12 +348,14 @@ int Solver::download(int ix, double* host, size_t n) {
     const long long P = I.npts, S = stride;
     if (pipe == NW_PIPE_PLAIN) {
         NW_TRY_HIP(hipMemcpy2DAsync(host, P * 8, d_state, S * 8, P * 8, 6, hipMemcpyDeviceToHost, stream));
-    } else if (prec == kFP64) {
-        const double* src = static_cast<const double*>(d_u[cur]) + I.node_base;
-        NW_TRY_HIP(hipMemcpy2DAsync(host, P * 8, src, S * 8, P * 8, 6, hipMemcpyDeviceToHost, stream));
     } else {
-        const float* src = static_cast<const float*>(d_u[cur]) + I.node_base;
-        NW_TRY_HIP((launch_convert<double, float>(d_stage, P, src, S, P, 6, stream)));
+        if (prec == kFP64) {
+            const double* src = static_cast<const double*>(d_u[cur]) + 6 * I.node_base;
+            NW_TRY_HIP((launch_convert<double, double>(d_stage, P, 1, src, 1, 6, P, 6, stream)));
+        } else {
+            const float* src = static_cast<const float*>(d_u[cur]) + 6 * I.node_base;
+            NW_TRY_HIP((launch_convert<double, float>(d_stage, P, 1, src, 1, 6, P, 6, stream)));
+        }
         NW_TRY_HIP(hipMemcpyAsync(host, d_stage, 8 * 6 * P, hipMemcpyDeviceToHost, stream));
     }
     NW_TRY_HIP(hipStreamSynchronize(stream));
@@ -400,19 +405,19 @@ int Solver::enqueue_fused_stage(int s, int parity) {
         const int nh = static_cast<int>(nhalo);
         if (prec == kFP64) {
             NW_TRY_HIP(launch_pack<double>(static_cast<const double*>(uin), static_cast<double*>(d_sendbuf), d_send_vol,
-                                           d_off, d_sizes, nh, stride, stream));
+                                           d_off, d_sizes, nh, 1, 6, stream));
         } else {
             NW_TRY_HIP(launch_pack<float>(static_cast<const float*>(uin), static_cast<float*>(d_sendbuf), d_send_vol,
-                                          d_off, d_sizes, nh, stride, stream));
+                                          d_off, d_sizes, nh, 1, 6, stream));
         }
         const size_t rsz = (prec == kFP64) ? 8 : 4;
         NW_TRY_HIP(hipMemcpyAsync(d_recvbuf, d_sendbuf, rsz * 6 * nhalo, hipMemcpyDeviceToDevice, stream));
         if (prec == kFP64) {
             NW_TRY_HIP(launch_unpack<double>(static_cast<const double*>(d_recvbuf), static_cast<double*>(uin), d_off,
-                                             d_sizes, nh, npts, stride, stream));
+                                             d_sizes, nh, npts, 1, 6, stream));
         } else {
             NW_TRY_HIP(launch_unpack<float>(static_cast<const float*>(d_recvbuf), static_cast<float*>(uin), d_off,
-                                            d_sizes, nh, npts, stride, stream));
+                                            d_sizes, nh, npts, 1, 6, stream));
         }
     }
     int st;
@@ -435,7 +440,7 @@ int Solver::enqueue_plain_stage(int s) {
     const long long S = stride;
     if (nhalo > 0) {
         const int nh = static_cast<int>(nhalo);
-        NW_TRY_HIP(launch_pack<double>(st, static_cast<double*>(d_sendbuf), d_send_vol, d_off, d_sizes, nh, S, stream));
+        NW_TRY_HIP(launch_pack<double>(st, static_cast<double*>(d_sendbuf), d_send_vol, d_off, d_sizes, nh, S, 1, stream));
         NW_TRY_HIP(hipMemcpyAsync(d_recvbuf, d_sendbuf, 8 * 6 * nhalo, hipMemcpyDeviceToDevice, stream));
     }
     PlainVolumeArgs v{};
@@ -452,7 +457,7 @@ int Solver::enqueue_plain_stage(int s) {
     NW_TRY_HIP(launch_volume_curl(v, stream));
     if (nhalo > 0) {
         NW_TRY_HIP(launch_unpack<double>(static_cast<const double*>(d_recvbuf), st, d_off, d_sizes,
-                                         static_cast<int>(nhalo), npts, S, stream));
+                                         static_cast<int>(nhalo), npts, S, 1, stream));
     }
     NW_TRY_HIP(launch_restrict(st, d_fEN, d_fHN, d_vM, tfp, S, stream));
     NW_TRY_HIP(launch_flux(st, d_fEN, d_fHN, d_vP, d_bc, d_face[0], d_face[1], d_face[2], d_flux, c0, tfp, S, d_vM,
@@ -586,12 +591,13 @@ int Solver::diag(int ix, nw_diag* out) {
     const ElemMat* mat = I.media ? d_mat + I.elem_base : nullptr;
     const ElemGeom* geom = d_geom + I.elem_base;
     if (pipe == NW_PIPE_PLAIN) {
-        NW_TRY_HIP(launch_diag<double>(d_state, stride, npts, n1, d_met[3], nullptr, mat, d_diag_part, d_diag_out, stream));
+        NW_TRY_HIP(launch_diag<double>(d_state, stride, 1, npts, n1, d_met[3], nullptr, mat, d_diag_part, d_diag_out,
+                                       stream));
     } else if (prec == kFP64) {
-        NW_TRY_HIP(launch_diag<double>(static_cast<const double*>(d_u[cur]) + I.node_base, stride, I.npts, I.n1,
+        NW_TRY_HIP(launch_diag<double>(static_cast<const double*>(d_u[cur]) + 6 * I.node_base, 1, 6, I.npts, I.n1,
                                        nullptr, geom, mat, d_diag_part, d_diag_out, stream));
     } else {
-        NW_TRY_HIP(launch_diag<float>(static_cast<const float*>(d_u[cur]) + I.node_base, stride, I.npts, I.n1,
+        NW_TRY_HIP(launch_diag<float>(static_cast<const float*>(d_u[cur]) + 6 * I.node_base, 1, 6, I.npts, I.n1,
                                       nullptr, geom, mat, d_diag_part, d_diag_out, stream));
     }
     double h[3];
