@@ -13,9 +13,11 @@ Prints one JSON line: DOF-updates/s of the FP32 fused path (value), the
 roofline of the fused stage kernel (algorithmic bytes per launch / mean launch
 time from HIP events on the solver's stream), the FP64 fused and FP64
 reference-decomposed ("plain") rates, the FP32 relative L2 error against the
-FP64 path after 1000 steps, the other workloads (c3 PEC 32^3 with media, c4
-256 batched cavities, c5 64^3 share with the halo stand-in) and the CPU
-baseline (the oracle's FP64 step on the host cores, bounded sample).
+FP64 path after 1000 steps with the energy drift of both, five repeats of the
+timed loop, the PEC-cube TM110 case (relative L2 to the analytic mode, energy
+drift), the other workloads (c3 PEC 32^3 with media, c4 256 batched cavities,
+c5 64^3 share with the halo stand-in) and the CPU baseline (the oracle's FP64
+step on the host cores, bounded sample, and one step on one thread).
 """
 import argparse
 import json
@@ -85,20 +87,23 @@ def rel_l2(a, b, w):
     return math.sqrt(num / den)
 
 
-def cpu_baseline(dt, budget_s=15.0):
+def cpu_baseline(dt, budget_s=12.0):
     """The oracle (FP64 CPU restatement of the reference step) on the host
-    cores, on a bounded sample of the headline workload."""
+    cores, on a bounded sample of the headline workload; plus one step on a
+    single thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
 
     threads = min(16, os.cpu_count() or 1)
-    po.set_threads(threads)
     o = po.Mesh.box(8, (16, 16, 16), (0, 1, 0, 1, 0, 1), (1, 1, 1), jitter=0.1, seed=2)
     x, y, z = o.x, o.y, o.zc
     zero = np.zeros_like(x)
     st = np.concatenate([zero, zero, np.exp(-((x - .5) ** 2 + (y - .5) ** 2 + (z - .5) ** 2) / 0.01), zero, zero, zero])
     s = po.Solver(o, 1.0)
     s.upload(st)
+    po.set_threads(1)
+    t_one = s.run(dt, 1)
+    po.set_threads(threads)
     t1 = s.run(dt, 1)  # one step sizes the sample
     n = max(1, min(1000, int(budget_s / max(t1, 1e-3))))
     t = s.run(dt, n)
@@ -112,7 +117,8 @@ def cpu_baseline(dt, budget_s=15.0):
         pass
     return {"value": 6 * o.npts * n / t, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
             "sample": "%d LSRK45 step(s) of the same 16^3 n1=8 jittered periodic mesh, FP64, %d OpenMP "
-                      "threads (%s), %.1f s" % (n, threads, cpu_model, t)}
+                      "threads (%s), %.1f s" % (n, threads, cpu_model, t),
+            "value_1_thread": 6 * o.npts / t_one}
 
 
 def load_traffic(name):
@@ -134,6 +140,27 @@ def profile_run(nw, W, precision, steps):
     ms = ctypes.c_double()
     nw._lib.check(nw.lib().nw_bench_copy(256 << 20, 3, ctypes.byref(ms)))
     print("profile run: %d steps (%d fused launches), 4 copies of 256 MiB" % (steps, 5 * steps), flush=True)
+
+
+def tm110_c1(nw, W):
+    """PEC unit cube 4^3, p = 4, CFL 0.5, TM110, 200 steps (the reference's CPU-runnable
+    case): relative L2 against the analytic eigenmode and energy drift, FP64 fused and
+    FP32 fused."""
+    wl = W.c1()
+    m = wl.meshes[0]
+    x, y, _ = m.coords()
+    w = m.getJac() * np.tile(m.getW3(), m.getNumElements())
+    exact = W.pack(W.tm_mode(x, y, wl.steps * wl.dt))
+    out = {"steps": wl.steps, "dt": wl.dt}
+    for prec in ("fp64", "fp32"):
+        s = make_solver(nw, wl, prec)
+        e0 = s.diagnostics()["energy"]
+        s.run(wl.dt, wl.steps)
+        e1 = s.diagnostics()["energy"]
+        got = s.downloadState()
+        s.finalize()
+        out[prec] = {"rel_l2_vs_analytic": rel_l2(got, exact, w), "energy_drift": (e1 - e0) / e0}
+    return out
 
 
 def other_config(nw, W, name, steps):
@@ -201,17 +228,25 @@ def main():
     v64 = dof * cmp_steps / (ms64 * 1e-3)
     v64p = dof * cmp_steps / (ms64p * 1e-3)
 
-    # stated error: the measured precision vs the FP64 path after 1000 steps
-    acc = None
+    # repeats of the timed loop (median reported beside the contract's single timing)
+    reps = [time_path(nw, wl, args.precision, "fused", args.steps, 2)[0] / args.steps for _ in range(5)]
+
+    # stated error: the measured precision vs the FP64 path after 1000 steps,
+    # and the energy drift (U_end - U_0) / U_0 of both (upwind: dissipative)
+    acc = drift = None
     if args.accuracy_steps > 0 and args.precision != "fp64":
-        outs = {}
+        outs, drift = {}, {}
         for prec in (args.precision, "fp64"):
             s = make_solver(nw, wl, prec)
+            e0 = s.diagnostics()["energy"]
             s.run(dt, args.accuracy_steps)
+            drift[prec] = (s.diagnostics()["energy"] - e0) / e0
             outs[prec] = s.downloadState()
             s.finalize()
         w = m.getJac() * np.tile(m.getW3(), nelt)  # dV = J w3 (3dboxpec_test.cpp:84)
         acc = rel_l2(outs[args.precision], outs["fp64"], w)
+
+    c1 = tm110_c1(nw, W)
 
     others = {}
     if args.configs and args.configs != "none":
@@ -248,6 +283,10 @@ def main():
         "speedup_vs_fp64_plain": value / v64p,
         "rel_l2_vs_fp64": acc,
         "rel_l2_steps": args.accuracy_steps,
+        "energy_drift": drift,
+        "ms_per_step_repeats": reps,
+        "ms_per_step_median": float(np.median(reps)),
+        "tm110_c1": c1,
         "other_configs": others,
         "wall_s_timed": wall,
         "cpu_baseline": cpu,
