@@ -166,7 +166,7 @@ typedef struct nw_dg_info {
 int nw_dg_get_info(const nw_dg* s, nw_dg_info* out);
 /* Extension (batched cavities): one solver advancing `count` independent meshes of
  * any mix of n1 with a common dt.  Instances of equal n1 form one segment; each
- * stage launches one fused kernel per segment, as concurrent graph branches.
+ * stage launches one persistent fused kernel per segment, one after another.
  * Instance i is addressed by its index in `meshes`. */
 int nw_batch_create(int count, const nw_mesh* const* meshes, double c0, const nw_opts* opts, nw_dg** out);
 int nw_dg_upload_instance(nw_dg* s, int instance, const double* host, size_t n);

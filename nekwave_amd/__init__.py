@@ -345,8 +345,8 @@ class DeviceBuffer:
 
 class BatchSolver(DgSolver):
     """Many independent meshes (any mix of n1, common dt) in one solver: each
-    stage launches one fused kernel per n1 segment, as concurrent graph
-    branches.  Instances are addressed by their index in the list given to
+    stage launches one persistent fused kernel per n1 segment, one after
+    another.  Instances are addressed by their index in the list given to
     initialize()."""
 
     def __init__(self, precision="fp32", use_graph=True, device=0, tile_option=-1):

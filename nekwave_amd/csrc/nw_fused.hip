@@ -320,6 +320,11 @@ void k_fused_stage(FusedArgs A) {
         put_rec(rec0, g0 & 1);
         fetch(g0, nb0, cur);
     }
+    // Wait for the prologue's reads here: left in flight, the compiler's
+    // memory-counter analysis merges them into the loop and then waits, at the
+    // first use of each iteration's prefetched values, on that iteration's
+    // most recent stores as well.
+    __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
     __syncthreads();
 
     for (int g = g0; g < g1; ++g) {

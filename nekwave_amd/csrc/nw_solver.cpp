@@ -53,12 +53,6 @@ void Solver::release() {
     }
     for (void* p : allocs) (void)hipFree(p);
     allocs.clear();
-    for (hipEvent_t e : fork_ev) (void)hipEventDestroy(e);
-    for (hipEvent_t e : join_ev) (void)hipEventDestroy(e);
-    fork_ev.clear();
-    join_ev.clear();
-    for (hipStream_t s : aux) (void)hipStreamDestroy(s);
-    aux.clear();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     ev0 = ev1 = nullptr;
@@ -248,18 +242,6 @@ int Solver::init(const Mesh* const* meshes, int count, double c0_, const nw_opts
             std::vector<float> w32(w64.begin(), w64.end());
             if ((st = upload_vec(reinterpret_cast<float**>(&d_fw), w32))) return st;
         }
-        for (size_t x = 1; x < seg.size(); ++x) {
-            hipStream_t s2;
-            NW_TRY_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
-            aux.push_back(s2);
-        }
-        for (size_t x = 0; x < seg.size(); ++x) {
-            hipEvent_t a, b;
-            NW_TRY_HIP(hipEventCreateWithFlags(&a, hipEventDisableTiming));
-            NW_TRY_HIP(hipEventCreateWithFlags(&b, hipEventDisableTiming));
-            fork_ev.push_back(a);
-            join_ev.push_back(b);
-        }
         kernels_per_step = 5 * (static_cast<int>(seg.size()) + (nhalo > 0 ? 2 : 0));
     }
     NW_TRY_HIP(hipStreamSynchronize(stream));
@@ -397,8 +379,7 @@ int Solver::launch_segment(const Segment& sg, int s, int parity, int option, hip
 }
 
 // One stage of the fused pipeline, reading d_u[parity] and writing d_u[1-parity].
-// Segments of different n1 are independent within a stage: with several, they
-// run as concurrent branches (forked from and joined back to the main stream).
+// Segments of different n1 are independent within a stage.
 int Solver::enqueue_fused_stage(int s, int parity) {
     void* uin = d_u[parity];
     if (nhalo > 0) {
@@ -420,18 +401,12 @@ int Solver::enqueue_fused_stage(int s, int parity) {
                                             d_sizes, nh, npts, 1, 6, stream));
         }
     }
+    // Segments (one per n1) run one after another on the stream: each launch is
+    // a persistent grid sized to fill the GPU, so concurrent segment launches
+    // would only compete for the same CUs.
     int st;
-    if (seg.size() == 1) return launch_segment(seg[0], s, parity, seg[0].option, stream);
-    NW_TRY_HIP(hipEventRecord(fork_ev[0], stream));
-    for (size_t x = 1; x < seg.size(); ++x) NW_TRY_HIP(hipStreamWaitEvent(aux[x - 1], fork_ev[0], 0));
-    for (size_t x = 0; x < seg.size(); ++x) {
-        hipStream_t sx = x == 0 ? stream : aux[x - 1];
-        if ((st = launch_segment(seg[x], s, parity, seg[x].option, sx))) return st;
-    }
-    for (size_t x = 1; x < seg.size(); ++x) {
-        NW_TRY_HIP(hipEventRecord(join_ev[x], aux[x - 1]));
-        NW_TRY_HIP(hipStreamWaitEvent(stream, join_ev[x], 0));
-    }
+    for (size_t x = 0; x < seg.size(); ++x)
+        if ((st = launch_segment(seg[x], s, parity, seg[x].option, stream))) return st;
     return NW_OK;
 }
 

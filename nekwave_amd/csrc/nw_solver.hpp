@@ -24,8 +24,6 @@ struct Segment {
 struct Solver {
     int device = -1;
     hipStream_t stream = nullptr;
-    std::vector<hipStream_t> aux;  // one per extra segment: concurrent branches of a stage
-    std::vector<hipEvent_t> fork_ev, join_ev;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int prec = 0, pipe = 1;
     bool use_graph = true, initialized = false, uploaded = false, media = false, autotune = false, tuned = false;
