@@ -25,7 +25,7 @@ HOST_SOURCES = ["nw_mesh.cpp", "nw_meshio.cpp", "nw_solver.cpp", "nw_api.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-Wall", "-Wno-unused-result",
           "-I" + os.path.join(ROOT, "include")]
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-ffp-contract=fast"]
+          "-ffp-contract=on"]
 
 
 def hipcc():

@@ -78,13 +78,13 @@ __device__ __forceinline__ void pin(T& v) {
 }
 
 // Occupancy target (waves per SIMD) that the register budget is sized for.
-// A workgroup of two waves per SIMD gets 4 (two workgroups per CU in 128
-// VGPRs); any other shape gets 3 (168 VGPRs), which keeps the prefetch
-// registers free of spills: a spill reload inside the loop would wait on every
-// prefetch in flight.
-template <int NT>
+// FP32 fields: a workgroup of two waves per SIMD gets 4 (two workgroups per CU
+// in 128 VGPRs), any other shape 3 (168 VGPRs).  FP64 fields get 2 (256
+// VGPRs): the doubled prefetch registers spill otherwise, and a spill reload
+// inside the loop waits on every prefetch in flight (measured 1.8x slower).
+template <int NT, int RBYTES>
 struct WavesPerEU {
-    static constexpr int value = ((NT + 63) / 64 + 3) / 4 == 2 ? 4 : 3;
+    static constexpr int value = RBYTES == 8 ? 2 : (((NT + 63) / 64 + 3) / 4 == 2 ? 4 : 3);
 };
 
 // Global traffic goes through raw buffer instructions: the base address lives
@@ -194,7 +194,7 @@ __device__ __forceinline__ void norm_and_inverse(double n2, double& len, double&
 //   4. lift: each node gathers its (at most three) face terms, one per
 //      direction, in fixed face order; inverse mass; LSRK update; stores.
 template <int N1, int EPW, bool FIRST, typename R, typename KR>
-__global__ __launch_bounds__(EPW* N1* N1* N1) __attribute__((amdgpu_waves_per_eu(WavesPerEU<EPW * N1 * N1 * N1>::value)))
+__global__ __launch_bounds__(EPW* N1* N1* N1) __attribute__((amdgpu_waves_per_eu(WavesPerEU<EPW * N1 * N1 * N1, sizeof(R)>::value)))
 void k_fused_stage(FusedArgs A) {
     constexpr int N2 = N1 * N1, NP = N1 * N2, NF = 6 * N2, NT = EPW * NP;
     constexpr int FPT = (EPW * NF + NT - 1) / NT;  // face points per thread (1 for n1 >= 6)
