@@ -106,6 +106,9 @@ template <typename T>
 __device__ __forceinline__ T bld(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
     return bload(static_cast<T*>(nullptr), r, vo, so);
 }
+__device__ __forceinline__ int bld_i32(__amdgpu_buffer_rsrc_t r, unsigned vo) {
+    return static_cast<int>(__builtin_amdgcn_raw_buffer_load_b32(r, vo, 0, 0));
+}
 __device__ __forceinline__ int4 bld_int4(__amdgpu_buffer_rsrc_t r, unsigned vo) {
     typedef __attribute__((ext_vector_type(4))) int i32x4;
     const i32x4 v = __builtin_bit_cast(i32x4, __builtin_amdgcn_raw_buffer_load_b128(r, vo, 0, 0));
@@ -182,8 +185,11 @@ __device__ __forceinline__ void norm_and_inverse(double n2, double& len, double&
     inv = len > 1e-14 ? 1.0 / len : 0.0;
 }
 
-// One workgroup owns a contiguous run of element groups (EPW elements each)
-// and walks it, one group per iteration; HBM reads are issued one or two
+// Persistent grid: in iteration t workgroup W takes the group at position
+// t * gstride + W of the group order, so one iteration of the grid covers a
+// contiguous window of the order (a compact block of elements when the order
+// follows a space-filling curve, keeping face neighbours' traces in L2) and
+// each XCD a contiguous part of that window.  HBM reads are issued one or two
 // iterations before they are needed, so they travel while earlier groups
 // compute.  Per group:
 //   1. FP64 geometry of the thread's node; the node's face (Nanson) vectors
@@ -212,13 +218,13 @@ void k_fused_stage(FusedArgs A) {
     __shared__ ElemRec sE[2][EPW];     // element records, double-buffered over groups
 
     // XCD-aware remap: the hardware deals workgroups round-robin over the 8
-    // XCDs; logical workgroups, and with them element runs, are contiguous
-    // per XCD so face neighbours are read from that XCD's L2.
+    // XCDs; logical workgroups are contiguous per XCD so each XCD's share of a
+    // window, and with it most face neighbours, stays in that XCD's L2.
     const int hb = blockIdx.x;
     const int W = (hb & 7) * A.wchunk + (hb >> 3);
-    const int g0 = W * A.gper;
-    const int g1 = min(g0 + A.gper, A.ngroups);
-    if (g0 >= g1) return;  // uniform over the block, before any barrier
+    if (W >= A.ngroups) return;  // uniform over the block, before any barrier
+    const int WT = A.gstride;
+    const int tl = (A.ngroups - 1 - W) / WT;  // this workgroup's last iteration
 
     const int tid = threadIdx.x;
     const int es = tid / NP;
@@ -226,7 +232,7 @@ void k_fused_stage(FusedArgs A) {
     const int i = m % N1, j = (m / N1) % N1, k = m / N2;
     const __amdgpu_buffer_rsrc_t rU = make_rsrc(A.uin), rO = make_rsrc(A.uout), rK = make_rsrc(A.kbuf);
     const __amdgpu_buffer_rsrc_t rW = make_rsrc(A.fw), rN = make_rsrc(A.fnb);
-    const __amdgpu_buffer_rsrc_t rR = make_rsrc(A.rec);
+    const __amdgpu_buffer_rsrc_t rR = make_rsrc(A.rec), rG = make_rsrc(A.gorder);
     constexpr unsigned UB = 6 * sizeof(R), KB = 6 * sizeof(KR);  // bytes per node of u and of k
 
     // loop-invariant: node coordinates, inverse mass weight
@@ -243,11 +249,14 @@ void k_fused_stage(FusedArgs A) {
     for (int x = tid; x < 6; x += NT) sF[EPW * NF][x] = R(0);
 
     auto clamp_e = [&](int g, int slot) { return min(g * EPW + slot, A.nelt - 1); };
+    // group of iteration t (past the last iteration: the last group again)
+    auto order_at = [&](int t) { return bld_i32(rG, unsigned(min(t, tl) * WT + W) * 4u); };
     auto node_of = [&](int g) -> unsigned {  // clamped: every read stays in bounds
         return static_cast<unsigned>(A.node_base + (long long)clamp_e(g, es) * NP + m);
     };
     // Every read an iteration depends on is issued ahead of it, so no wave
     // waits on memory latency inside an iteration:
+    //   three iterations ahead: the group order entry;
     //   two groups ahead: the face-neighbour table rows (they address the
     //   neighbour-trace gather) and the element records (staged into LDS one
     //   iteration ahead);
@@ -306,19 +315,24 @@ void k_fused_stage(FusedArgs A) {
         }
     };
 
-    // prologue: records of g0 in LDS, fields of g0 in flight, two-ahead reads of g0 + 1
+    // prologue: records of the first group in LDS, its fields in flight,
+    // two-ahead reads of the second
     Pre cur;
     Nbr nb1;
     Rec rec1;
+    // groups of iterations t, t + 1, t + 2 (wave-uniform)
+    int gA = __builtin_amdgcn_readfirstlane(order_at(0));
+    int gB = __builtin_amdgcn_readfirstlane(order_at(1));
+    int gC = __builtin_amdgcn_readfirstlane(order_at(2));
     {
         Nbr nb0;
         Rec rec0;
-        fetch_nbr(g0, nb0);
-        fetch_rec(g0, rec0);
-        fetch_nbr(min(g0 + 1, g1 - 1), nb1);
-        fetch_rec(min(g0 + 1, g1 - 1), rec1);
-        put_rec(rec0, g0 & 1);
-        fetch(g0, nb0, cur);
+        fetch_nbr(gA, nb0);
+        fetch_rec(gA, rec0);
+        fetch_nbr(gB, nb1);
+        fetch_rec(gB, rec1);
+        put_rec(rec0, 0);
+        fetch(gA, nb0, cur);
     }
     // Wait for the prologue's reads here: left in flight, the compiler's
     // memory-counter analysis merges them into the loop and then waits, at the
@@ -327,11 +341,13 @@ void k_fused_stage(FusedArgs A) {
     __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
     __syncthreads();
 
-    for (int g = g0; g < g1; ++g) {
+    for (int t = 0; t <= tl; ++t) {
+        const int g = gA;
         Nbr nb2;
         Rec rec2;
-        fetch_nbr(min(g + 2, g1 - 1), nb2);
-        fetch_rec(min(g + 2, g1 - 1), rec2);
+        fetch_nbr(gC, nb2);
+        fetch_rec(gC, rec2);
+        const int gnext = order_at(t + 3);
         // reads of this group needed late (LSRK registers, flux weights) go
         // before the prefetch: waiting on them does not wait on it
         const unsigned nd = node_of(g);
@@ -346,9 +362,9 @@ void k_fused_stage(FusedArgs A) {
             ld4(rW, face * unsigned(4 * sizeof(R)), fw[t]);
         }
         Pre nxt;
-        fetch(min(g + 1, g1 - 1), nb1, nxt);  // in flight while this group computes
+        fetch(gB, nb1, nxt);  // in flight while this group computes
 
-        const ElemRec& ER = sE[g & 1][es];
+        const ElemRec& ER = sE[t & 1][es];
 
         // ---- 1. FP64 geometry: cofactors C = J J^-T (mesh.cpp:291-309 times J) ----
         R Crx, Cry, Crz, Csx, Csy, Csz, Ctx, Cty, Ctz, iJ;
@@ -385,7 +401,7 @@ void k_fused_stage(FusedArgs A) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) sU[es][c][m] = cur.u[c];
         __syncthreads();
-        put_rec(rec1, (g + 1) & 1);  // read from the next iteration on, after the barrier below
+        put_rec(rec1, (t + 1) & 1);  // read from the next iteration on, after the barrier below
 
         // ---- 2. volume: reference derivatives (local_grad3, dg_kernels.cu:101-124)
         // folded straight into J * curl (dg_kernels.cu:140-149) ----
@@ -503,6 +519,9 @@ void k_fused_stage(FusedArgs A) {
         cur = nxt;
         nb1 = nb2;
         rec1 = rec2;
+        gA = gB;
+        gB = gC;
+        gC = __builtin_amdgcn_readfirstlane(gnext);
     }
 }
 
@@ -544,11 +563,11 @@ hipError_t launch_one(const FusedArgs& a, hipStream_t s) {
     if (span * sizeof(double) >= (1ull << 32)) return hipErrorInvalidValue;
     FusedArgs b = a;
     b.ngroups = (a.nelt + EPW - 1) / EPW;
+    if (!a.gorder) return hipErrorInvalidValue;
     const int want = b.ngroups < resident ? b.ngroups : resident;
-    b.gper = (b.ngroups + want - 1) / want;
-    const int used = (b.ngroups + b.gper - 1) / b.gper;
-    b.wchunk = (used + 7) / 8;
-    const dim3 grid(8 * b.wchunk), block(NT);
+    b.wchunk = (want + 7) / 8;
+    b.gstride = 8 * b.wchunk;  // workgroups past ngroups return at once
+    const dim3 grid(b.gstride), block(NT);
     if (a.first_stage)
         hipLaunchKernelGGL((k_fused_stage<N1, EPW, true, R, KR>), grid, block, 0, s, b);
     else

@@ -21,8 +21,9 @@ struct FusedArgs {
     int nelt;                 // elements in this segment
     int elem_base;            // global index of the segment's first element (into mat)
     int ngroups;              // element groups (EPW elements each); set by the launcher
-    int gper;                 // groups per workgroup (persistent loop)
-    int wchunk;               // workgroups per XCD (XCD-aware remap)
+    int gstride;              // workgroups of the persistent grid; set by the launcher
+    int wchunk;               // workgroups per XCD (XCD-aware remap); set by the launcher
+    const int* gorder;        // ngroups entries: the order groups are visited in (a permutation)
     const ElemGeom* geom;     // per element, segment-local
     const double* rec;        // per element, segment-local: 24 trilinear coefficients, 1/eps, 1/mu
     const ElemMat* mat;       // per element, global index (always present; vacuum = 1)

@@ -4,6 +4,7 @@
 #include "nw_solver.hpp"
 
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -38,6 +39,48 @@ static const double kRkA[5] = {0.0, -567301805773.0 / 1357537059087.0, -24042679
 static const double kRkB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
                                1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
                                2277821191437.0 / 14882151754819.0};
+
+// Visiting order of the element groups (EPW consecutive elements each) of one
+// fused-kernel launch.  For a single mesh: a Morton (Z-order) curve over the
+// groups' first-element centroids, so the window of groups the persistent grid
+// covers in one iteration is a compact block of elements and most face
+// neighbours are read while still in L2.  For a batch (m == nullptr) the
+// storage order: instances are small and independent.
+static void append_group_order(const Mesh* m, int epw, int nelt, std::vector<int>& out) {
+    const int ng = (nelt + epw - 1) / epw;
+    const size_t base = out.size();
+    out.resize(base + size_t(ng));
+    std::iota(out.begin() + base, out.end(), 0);
+    if (!m || ng < 2) return;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int g = 0; g < ng; ++g)
+        for (int d = 0; d < 3; ++d) {
+            const double x = m->geom[size_t(g) * epw].c[d][0];  // centroid of the trilinear map
+            lo[d] = std::min(lo[d], x);
+            hi[d] = std::max(hi[d], x);
+        }
+    auto spread = [](uint64_t v) {  // 10 bits -> every third bit
+        v &= 0x3ff;
+        v = (v | (v << 16)) & 0x030000ffull;
+        v = (v | (v << 8)) & 0x0300f00full;
+        v = (v | (v << 4)) & 0x030c30c3ull;
+        v = (v | (v << 2)) & 0x09249249ull;
+        return v;
+    };
+    std::vector<uint64_t> key(static_cast<size_t>(ng));
+    for (int g = 0; g < ng; ++g) {
+        uint64_t code = 0;
+        for (int d = 0; d < 3; ++d) {
+            const double span = hi[d] - lo[d];
+            const double x = span > 0 ? (m->geom[size_t(g) * epw].c[d][0] - lo[d]) / span : 0.0;
+            const uint64_t q = static_cast<uint64_t>(std::min(1023.0, std::max(0.0, x * 1024.0)));
+            code |= spread(q) << d;
+        }
+        key[size_t(g)] = (code << 32) | static_cast<uint64_t>(g);
+    }
+    std::sort(key.begin(), key.end());
+    for (int g = 0; g < ng; ++g) out[base + size_t(g)] = static_cast<int>(key[size_t(g)] & 0xffffffffull);
+}
 
 Solver::~Solver() { release(); }
 
@@ -222,6 +265,13 @@ int Solver::init(const Mesh* const* meshes, int count, double c0_, const nw_opts
             }
         }
         if ((st = upload_vec(&d_fnb, fnb))) return st;
+        std::vector<int> gorder;
+        for (int opt = 0; opt < 2; ++opt)
+            for (Segment& sg : seg) {
+                sg.goff[opt] = static_cast<long long>(gorder.size());
+                append_group_order(count == 1 ? meshes[0] : nullptr, fused_epw(sg.n1, opt, prec), sg.nelt, gorder);
+            }
+        if ((st = upload_vec(&d_gorder, gorder))) return st;
         // flux weights per element face from the two media (the plus side of a
         // mirror face is the element itself; of a cut face, its ghost medium)
         std::vector<double> w64(size_t(nelt) * 24);
@@ -368,6 +418,7 @@ int Solver::launch_segment(const Segment& sg, int s, int parity, int option, hip
     a.mat = d_mat;
     a.gmat = d_gmat;
     a.fnb = d_fnb + size_t(sg.elem_base) * 6;
+    a.gorder = d_gorder + sg.goff[option ? 1 : 0];
     a.dt = d_dt;
     a.a = kRkA[s];
     a.b = kRkB[s];

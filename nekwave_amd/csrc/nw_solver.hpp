@@ -19,6 +19,7 @@ struct Instance {
 struct Segment {
     int n1 = 0, nelt = 0, elem_base = 0, option = 0;
     long long node_base = 0;
+    long long goff[2] = {0, 0};  // offset of this segment's group order, per tile option
 };
 
 struct Solver {
@@ -52,6 +53,7 @@ struct Solver {
     int4* d_fnb = nullptr;
     void* d_fw = nullptr;  // flux weights per element face, 4 values of the field type
     double* d_rec = nullptr;  // per element: 24 trilinear coefficients, 1/eps, 1/mu
+    int* d_gorder = nullptr;  // per segment and tile option: the order the fused kernel visits groups in
     // plain pipeline (reference residency, single mesh)
     double *d_state = nullptr, *d_kk = nullptr, *d_rhs = nullptr, *d_D = nullptr, *d_w3 = nullptr;
     double* d_met[13] = {};  // x y z jac rx sx tx ry sy ty rz sz tz
