@@ -115,8 +115,11 @@ __device__ __forceinline__ int4 bld_int4(__amdgpu_buffer_rsrc_t r, unsigned vo) 
     return make_int4(v[0], v[1], v[2], v[3]);
 }
 // The six field values of one node (node-major state: 6 consecutive values
-// per node).  Accesses stay naturally aligned: FP32 nodes sit 24 bytes apart,
-// so three 8-byte accesses; FP64 nodes 48 bytes apart, so three 16-byte ones.
+// per node) in as few accesses as the node's alignment allows: FP32 nodes sit
+// 24 bytes apart (one 16-byte plus one 8-byte access; buffer accesses need only
+// dword alignment), FP64 nodes 48 bytes apart (three 16-byte accesses).  Two
+// instructions instead of three matter most for the neighbour-trace gathers,
+// whose lanes each touch a different cache line.
 typedef __attribute__((ext_vector_type(2))) unsigned u32x2;
 typedef __attribute__((ext_vector_type(4))) unsigned u32x4;
 typedef __attribute__((ext_vector_type(2))) float f32x2;
@@ -126,12 +129,9 @@ typedef __attribute__((ext_vector_type(2))) double f64x2;
 // 32-bit lanes of the loaded vector before the reinterpretation let the
 // compiler narrow the access to its first lane.)
 __device__ __forceinline__ void ld6(__amdgpu_buffer_rsrc_t r, unsigned vo, float (&o)[6]) {
-#pragma unroll
-    for (int x = 0; x < 3; ++x) {
-        const f32x2 a = __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(r, vo + 8u * x, 0, 0));
-        o[2 * x] = a[0];
-        o[2 * x + 1] = a[1];
-    }
+    const f32x4 a = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, vo, 0, 0));
+    const f32x2 b = __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(r, vo + 16u, 0, 0));
+    o[0] = a[0]; o[1] = a[1]; o[2] = a[2]; o[3] = a[3]; o[4] = b[0]; o[5] = b[1];
 }
 __device__ __forceinline__ void ld6(__amdgpu_buffer_rsrc_t r, unsigned vo, double (&o)[6]) {
 #pragma unroll
@@ -155,11 +155,10 @@ __device__ __forceinline__ void ld4(__amdgpu_buffer_rsrc_t r, unsigned vo, doubl
     }
 }
 __device__ __forceinline__ void st6(__amdgpu_buffer_rsrc_t r, unsigned vo, const float (&v)[6]) {
-#pragma unroll
-    for (int x = 0; x < 3; ++x) {
-        const f32x2 a = {v[2 * x], v[2 * x + 1]};
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, a), r, vo + 8u * x, 0, 0);
-    }
+    const f32x4 a = {v[0], v[1], v[2], v[3]};
+    const f32x2 b = {v[4], v[5]};
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, a), r, vo, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, b), r, vo + 16u, 0, 0);
 }
 __device__ __forceinline__ void st6(__amdgpu_buffer_rsrc_t r, unsigned vo, const double (&v)[6]) {
 #pragma unroll
