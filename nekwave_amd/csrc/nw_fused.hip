@@ -191,8 +191,8 @@ __device__ __forceinline__ void norm_and_inverse(double n2, double& len, double&
 // each XCD a contiguous part of that window.  HBM reads are issued one or two
 // iterations before they are needed, so they travel while earlier groups
 // compute.  Per group:
-//   1. FP64 geometry of the thread's node; the node's face (Nanson) vectors
-//      and its field values go to LDS;                              barrier
+//   1. the node's face (Nanson) vectors from its FP64 geometry (computed
+//      during the previous iteration) and its field values go to LDS; barrier
 //   2. volume: the three reference derivatives of each field from LDS folded
 //      straight into J * curl;
 //   3. face point (f, p, q): jumps, flux, dA -> LDS;                barrier
@@ -340,6 +340,25 @@ void k_fused_stage(FusedArgs A) {
     __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
     __syncthreads();
 
+    // FP64 geometry of the thread's node from an element record: cofactors
+    // C = J J^-T (mesh.cpp:291-309 times J) and 1/J, rounded to R.  A group's
+    // geometry is computed at the end of the previous iteration, where the
+    // waves otherwise wait for the prefetched fields to land.
+    auto geometry = [&](const ElemRec& ER, R(&C)[10]) {
+        const Tangents T = tangents(&ER.g, zr, zs, zt);
+        double Cr[3], Cs[3], Ct[3];
+        cross3(T.s, T.t, Cr);  // (rx, ry, rz) * J
+        cross3(T.t, T.r, Cs);  // (sx, sy, sz) * J
+        cross3(T.r, T.s, Ct);  // (tx, ty, tz) * J
+        const double det = T.r[0] * Cr[0] + T.r[1] * Cr[1] + T.r[2] * Cr[2];
+        C[0] = R(Cr[0]); C[1] = R(Cr[1]); C[2] = R(Cr[2]);
+        C[3] = R(Cs[0]); C[4] = R(Cs[1]); C[5] = R(Cs[2]);
+        C[6] = R(Ct[0]); C[7] = R(Ct[1]); C[8] = R(Ct[2]);
+        C[9] = recip(R(det));
+    };
+    R geo[10];
+    geometry(sE[0][es], geo);
+
     for (int t = 0; t <= tl; ++t) {
         const int g = gA;
         Nbr nb2;
@@ -365,20 +384,9 @@ void k_fused_stage(FusedArgs A) {
 
         const ElemRec& ER = sE[t & 1][es];
 
-        // ---- 1. FP64 geometry: cofactors C = J J^-T (mesh.cpp:291-309 times J) ----
-        R Crx, Cry, Crz, Csx, Csy, Csz, Ctx, Cty, Ctz, iJ;
-        {
-            const Tangents T = tangents(&ER.g, zr, zs, zt);
-            double Cr[3], Cs[3], Ct[3];
-            cross3(T.s, T.t, Cr);  // (rx, ry, rz) * J
-            cross3(T.t, T.r, Cs);  // (sx, sy, sz) * J
-            cross3(T.r, T.s, Ct);  // (tx, ty, tz) * J
-            const double det = T.r[0] * Cr[0] + T.r[1] * Cr[1] + T.r[2] * Cr[2];
-            Crx = R(Cr[0]); Cry = R(Cr[1]); Crz = R(Cr[2]);
-            Csx = R(Cs[0]); Csy = R(Cs[1]); Csz = R(Cs[2]);
-            Ctx = R(Ct[0]); Cty = R(Ct[1]); Ctz = R(Ct[2]);
-            iJ = recip(R(det));
-        }
+        // ---- 1. this group's geometry (computed at the end of the previous iteration) ----
+        const R Crx = geo[0], Cry = geo[1], Crz = geo[2], Csx = geo[3], Csy = geo[4], Csz = geo[5];
+        const R Ctx = geo[6], Cty = geo[7], Ctz = geo[8], iJ = geo[9];
         // The Nanson vector of a face point is the cofactor row of its face's
         // direction at that node (mesh.cpp:1011-1038: -+J (rx, ry, rz) etc.),
         // outward: r-faces +-C_r, s-faces +-C_s, t-faces +-C_t.
@@ -515,6 +523,7 @@ void k_fused_stage(FusedArgs A) {
             st6(rK, nd * KB, kn);
             st6(rO, nd * UB, un);
         }
+        geometry(sE[(t + 1) & 1][es], geo);  // the next group's record, staged in this iteration
         cur = nxt;
         nb1 = nb2;
         rec1 = rec2;
