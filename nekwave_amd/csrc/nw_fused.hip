@@ -231,7 +231,7 @@ void k_fused_stage(FusedArgs A) {
     const int i = m % N1, j = (m / N1) % N1, k = m / N2;
     const __amdgpu_buffer_rsrc_t rU = make_rsrc(A.uin), rO = make_rsrc(A.uout), rK = make_rsrc(A.kbuf);
     const __amdgpu_buffer_rsrc_t rW = make_rsrc(A.fw), rN = make_rsrc(A.fnb);
-    const __amdgpu_buffer_rsrc_t rR = make_rsrc(A.rec), rG = make_rsrc(A.gorder);
+    const __amdgpu_buffer_rsrc_t rR = make_rsrc(A.rec);
     constexpr unsigned UB = 6 * sizeof(R), KB = 6 * sizeof(KR);  // bytes per node of u and of k
 
     // loop-invariant: node coordinates, inverse mass weight
@@ -249,7 +249,19 @@ void k_fused_stage(FusedArgs A) {
 
     auto clamp_e = [&](int g, int slot) { return min(g * EPW + slot, A.nelt - 1); };
     // group of iteration t (past the last iteration: the last group again)
-    auto order_at = [&](int t) { return bld_i32(rG, unsigned(min(t, tl) * WT + W) * 4u); };
+    // Wave-uniform reads go through the scalar unit instead of the vector
+    // memory pipeline (a 64-lane load of one 16-byte row costs the address
+    // unit as much as 1 KiB of field data): the group order always, and at
+    // n1 = 8 with one element per workgroup, where wave w owns exactly the
+    // face points of face w, the face-neighbour row and the flux weights.
+    // They are issued at the top of an iteration and first waited on at its
+    // first barrier, before which the iteration reads no LDS.
+    constexpr bool SF = N2 == 64 && EPW == 1;
+    typedef const __attribute__((address_space(4))) int CInt;
+    const int wface = min(__builtin_amdgcn_readfirstlane(tid >> 6), 5);  // SF: this wave's face
+    CInt* const sGo = (CInt*)A.gorder;
+    CInt* const sNb = (CInt*)A.fnb;
+    auto order_s = [&](int t) -> int { return sGo[min(t, tl) * WT + W]; };
     auto node_of = [&](int g) -> unsigned {  // clamped: every read stays in bounds
         return static_cast<unsigned>(A.node_base + (long long)clamp_e(g, es) * NP + m);
     };
@@ -278,7 +290,13 @@ void k_fused_stage(FusedArgs A) {
         for (int t = 0; t < FPT; ++t) {
             const int fp = tid + t * NT;
             const int efs = min(fp / NF, EPW - 1), f = (fp - (fp / NF) * NF) / N2;
-            N.b[t] = bld_int4(rN, unsigned(clamp_e(g, efs) * 6 + f) * 16u);
+            if constexpr (SF)
+            {
+                CInt* r = sNb + (clamp_e(g, 0) * 6 + wface) * 4;
+                N.b[t] = make_int4(r[0], r[1], r[2], r[3]);
+            }
+            else
+                N.b[t] = bld_int4(rN, unsigned(clamp_e(g, efs) * 6 + f) * 16u);
         }
     };
     auto fetch_rec = [&](int g, Rec& Q) {
@@ -320,9 +338,7 @@ void k_fused_stage(FusedArgs A) {
     Nbr nb1;
     Rec rec1;
     // groups of iterations t, t + 1, t + 2 (wave-uniform)
-    int gA = __builtin_amdgcn_readfirstlane(order_at(0));
-    int gB = __builtin_amdgcn_readfirstlane(order_at(1));
-    int gC = __builtin_amdgcn_readfirstlane(order_at(2));
+    int gA = order_s(0), gB = order_s(1), gC = order_s(2);
     {
         Nbr nb0;
         Rec rec0;
@@ -365,7 +381,7 @@ void k_fused_stage(FusedArgs A) {
         Rec rec2;
         fetch_nbr(gC, nb2);
         fetch_rec(gC, rec2);
-        const int gnext = order_at(t + 3);
+        const int gnext = order_s(t + 3);
         // reads of this group needed late (LSRK registers, flux weights) go
         // before the prefetch: waiting on them does not wait on it
         const unsigned nd = node_of(g);
@@ -377,7 +393,14 @@ void k_fused_stage(FusedArgs A) {
             const int fp = tid + t * NT;
             const int efs = min(fp / NF, EPW - 1), f = (fp - (fp / NF) * NF) / N2;
             const unsigned face = unsigned(clamp_e(g, efs) * 6 + f);
-            ld4(rW, face * unsigned(4 * sizeof(R)), fw[t]);
+            if constexpr (SF) {
+                typedef const __attribute__((address_space(4))) R CR;
+                CR* w = (CR*)A.fw + size_t(clamp_e(g, 0) * 6 + wface) * 4;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) fw[t][c] = w[c];
+            } else {
+                ld4(rW, face * unsigned(4 * sizeof(R)), fw[t]);
+            }
         }
         Pre nxt;
         fetch(gB, nb1, nxt);  // in flight while this group computes
@@ -529,7 +552,7 @@ void k_fused_stage(FusedArgs A) {
         rec1 = rec2;
         gA = gB;
         gB = gC;
-        gC = __builtin_amdgcn_readfirstlane(gnext);
+        gC = gnext;
     }
 }
 
