@@ -198,7 +198,7 @@ __device__ __forceinline__ void norm_and_inverse(double n2, double& len, double&
 //   3. face point (f, p, q): jumps, flux, dA -> LDS;                barrier
 //   4. lift: each node gathers its (at most three) face terms, one per
 //      direction, in fixed face order; inverse mass; LSRK update; stores.
-template <int N1, int EPW, bool FIRST, typename R, typename KR>
+template <int N1, int EPW, bool FIRST, typename R, typename KR, bool MX>
 __global__ __launch_bounds__(EPW* N1* N1* N1) __attribute__((amdgpu_waves_per_eu(WavesPerEU<EPW * N1 * N1 * N1, sizeof(R)>::value)))
 void k_fused_stage(FusedArgs A) {
     constexpr int N2 = N1 * N1, NP = N1 * N2, NF = 6 * N2, NT = EPW * NP;
@@ -210,11 +210,19 @@ void k_fused_stage(FusedArgs A) {
         double ieps, imu;
     };
     static_assert(sizeof(ElemRec) == ERW * sizeof(double), "element record layout");
+    // MX: the volume contractions on the matrix cores (n1 = 8, FP32 fields,
+    // one element per workgroup), else on the vector ALUs.
+    static_assert(!MX || (N1 == 8 && EPW == 1 && sizeof(R) == 4), "matrix-core volume shape");
     __shared__ R sU[EPW][6][NP];
     __shared__ R sF[EPW * NF + 1][6];  // face terms dA*F; the last row stays zero
     __shared__ R sC[EPW * NF][9];      // per face point: Nanson vector, then the minus-side trace (6)
     __shared__ R sD[N2];               // D, row-major: sD[i * N1 + l] = D_il
     __shared__ ElemRec sE[2][EPW];     // element records, double-buffered over groups
+    // matrix-core path: the 18 reference derivatives of every node, by direction
+    // r (node order), s (line (i, k) major, j fastest), t (line (i, j) major, k
+    // fastest, lines 9 words apart so the node-order reads are conflict-free)
+    __shared__ R sA[MX ? 2 : 1][MX ? 6 : 1][MX ? NP : 1];
+    __shared__ R sAt[MX ? 6 : 1][MX ? 9 * N2 : 1];
 
     // XCD-aware remap: the hardware deals workgroups round-robin over the 8
     // XCDs; logical workgroups are contiguous per XCD so each XCD's share of a
@@ -436,6 +444,52 @@ void k_fused_stage(FusedArgs A) {
         // ---- 2. volume: reference derivatives (local_grad3, dg_kernels.cu:101-124)
         // folded straight into J * curl (dg_kernels.cu:140-149) ----
         R cE[3] = {R(0), R(0), R(0)}, cH[3] = {R(0), R(0), R(0)};
+        if constexpr (MX) {
+            // 36 tasks (direction d, field c, half of the 64 lines), in pairs:
+            // one v_mfma_f32_4x4x1_16b per column q of D and task.  Lane 4b + x
+            // supplies A = D[4h + x][q] and B = u(line 4(b >> 1) + x, q) of block
+            // b (h = b & 1 picks the output rows) and receives the derivative of
+            // that line at rows 4h..4h+3; each step is one fused multiply-add,
+            // so the result equals the vector contraction bit for bit.  Waves
+            // 0-5, which also compute face terms, take two pairs, waves 6, 7 three.
+            typedef __attribute__((ext_vector_type(4))) float v4f;
+            const int lane = tid & 63, wv = tid >> 6;
+            const int h = (lane >> 2) & 1;
+            const int lq = 4 * (lane >> 3) + (lane & 3);
+            R Ar[N1];
+#pragma unroll
+            for (int q = 0; q < N1; ++q) Ar[q] = sD[(4 * h + (lane & 3)) * N1 + q];
+            const int p0 = wv < 6 ? 2 * wv : 12 + 3 * (wv - 6), np = wv < 6 ? 2 : 3;
+            for (int pp = 0; pp < np; ++pp) {
+                const int P = p0 + pp, d = P / 6, c = P - 6 * d;
+                const R* u = sU[0][c];
+                R B0[N1], B1[N1];
+                const int l0 = lq, l1 = lq + 32;
+                if (d == 0) {
+#pragma unroll
+                    for (int q = 0; q < N1; ++q) { B0[q] = u[8 * l0 + q]; B1[q] = u[8 * l1 + q]; }
+                } else if (d == 1) {
+#pragma unroll
+                    for (int q = 0; q < N1; ++q) { B0[q] = u[(l0 & 7) + 64 * (l0 >> 3) + 8 * q]; B1[q] = u[(l1 & 7) + 64 * (l1 >> 3) + 8 * q]; }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < N1; ++q) { B0[q] = u[l0 + 64 * q]; B1[q] = u[l1 + 64 * q]; }
+                }
+                v4f a0 = {0.0f, 0.0f, 0.0f, 0.0f}, a1 = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int q = 0; q < N1; ++q) {
+                    a0 = __builtin_amdgcn_mfma_f32_4x4x1f32(Ar[q], B0[q], a0, 0, 0, 0);
+                    a1 = __builtin_amdgcn_mfma_f32_4x4x1f32(Ar[q], B1[q], a1, 0, 0, 0);
+                }
+                if (d < 2) {
+                    *reinterpret_cast<v4f*>(&sA[d][c][8 * l0 + 4 * h]) = a0;
+                    *reinterpret_cast<v4f*>(&sA[d][c][8 * l1 + 4 * h]) = a1;
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) { sAt[c][9 * l0 + 4 * h + r] = a0[r]; sAt[c][9 * l1 + 4 * h + r] = a1[r]; }
+                }
+            }
+        } else {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {  // reference direction r, s, t: one row of D live at a time
             const int row = d == 0 ? i : (d == 1 ? j : k);
@@ -465,6 +519,7 @@ void k_fused_stage(FusedArgs A) {
             // keep each direction's contraction here: sinking it into the update
             // below would hold the fields' LDS values in registers
             pin(cE[0]); pin(cE[1]); pin(cE[2]); pin(cH[0]); pin(cH[1]); pin(cH[2]);
+        }
         }
 
         // ---- 3. face terms dA * F of this thread's face point(s) ----
@@ -509,6 +564,25 @@ void k_fused_stage(FusedArgs A) {
             }
         }
         __syncthreads();  // face terms complete
+        if constexpr (MX) {  // fold the node's 18 derivatives into J * curl
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const R Cx = d == 0 ? Crx : (d == 1 ? Csx : Ctx);
+                const R Cy = d == 0 ? Cry : (d == 1 ? Csy : Cty);
+                const R Cz = d == 0 ? Crz : (d == 1 ? Csz : Ctz);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    R a[3];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        a[c] = d == 0 ? sA[0][3 * h + c][m] : (d == 1 ? sA[1][3 * h + c][j + 8 * i + 64 * k] : sAt[3 * h + c][k + 9 * (i + 8 * j)]);
+                    R* cv = h == 0 ? cE : cH;
+                    cv[1] += a[0] * Cz; cv[2] -= a[0] * Cy;
+                    cv[0] -= a[1] * Cz; cv[2] += a[1] * Cx;
+                    cv[0] += a[2] * Cy; cv[1] -= a[2] * Cx;
+                }
+            }
+        }
 
         // ---- 4. lift: one face term per direction (the zero row off the faces).
         // A node meets at most one face per direction; the sum (s + r) + t equals
@@ -558,8 +632,10 @@ void k_fused_stage(FusedArgs A) {
 
 // Elements per workgroup, two options per n1 (index 0 is the default), capped
 // so one workgroup's LDS stays within 64 KiB (two or more workgroups per CU).
+// At n1 = 8 option 1 is instead the matrix-core volume for FP32 fields.
 constexpr int kEpw[kMaxN1 + 1][2] = {
-    {1, 1}, {1, 1}, {8, 32}, {9, 4}, {4, 1}, {2, 1}, {1, 3}, {1, 2}, {1, 2}, {1, 1}, {1, 1}};
+    {1, 1}, {1, 1}, {8, 32}, {9, 4}, {4, 1}, {2, 1}, {1, 3}, {1, 2}, {1, 1}, {1, 1}, {1, 1}};
+constexpr bool matrix_volume(int n1, int opt, int rsz) { return n1 == 8 && opt == 1 && rsz == 4; }
 
 constexpr int lds_per_elem(int n1, int rsz) { return (6 * n1 * n1 * n1 + 36 * n1 * n1) * rsz; }
 constexpr int epw_of(int n1, int opt, int rsz) {
@@ -584,9 +660,10 @@ int resident_groups(K kernel, int threads) {
 template <int N1, int OPT, typename R, typename KR>
 hipError_t launch_one(const FusedArgs& a, hipStream_t s) {
     constexpr int EPW = epw_of(N1, OPT, sizeof(R));
+    constexpr bool MX = matrix_volume(N1, OPT, sizeof(R));
     constexpr int NT = EPW * N1 * N1 * N1;
     static int resident = 0;
-    if (resident <= 0) resident = resident_groups(k_fused_stage<N1, EPW, false, R, KR>, NT);
+    if (resident <= 0) resident = resident_groups(k_fused_stage<N1, EPW, false, R, KR, MX>, NT);
     if (resident <= 0) return hipErrorInvalidValue;
     // every byte offset (component block included) of a field, register or
     // ghost slot fits in 32 bits
@@ -600,9 +677,9 @@ hipError_t launch_one(const FusedArgs& a, hipStream_t s) {
     b.gstride = 8 * b.wchunk;  // workgroups past ngroups return at once
     const dim3 grid(b.gstride), block(NT);
     if (a.first_stage)
-        hipLaunchKernelGGL((k_fused_stage<N1, EPW, true, R, KR>), grid, block, 0, s, b);
+        hipLaunchKernelGGL((k_fused_stage<N1, EPW, true, R, KR, MX>), grid, block, 0, s, b);
     else
-        hipLaunchKernelGGL((k_fused_stage<N1, EPW, false, R, KR>), grid, block, 0, s, b);
+        hipLaunchKernelGGL((k_fused_stage<N1, EPW, false, R, KR, MX>), grid, block, 0, s, b);
     return hipGetLastError();
 }
 

@@ -25,6 +25,7 @@ CASES = {
     "periodic_jitter_p3": (4, (3, 3, 3), (0, 1, 0, 1, 0, 1), (1, 1, 1), 0.1, 2, False, False, 1.0),
     "periodic_jitter_p7": (8, (2, 2, 2), (0, 1, 0, 1, 0, 1), (1, 1, 1), 0.1, 2, False, False, 1.0),
     "media_pec_p5": (6, (3, 2, 2), (0, 1.5, 0, 1, 0, 1), (0, 0, 0), 0.0, 0, True, False, 1.0),
+    "media_pec_p7": (8, (3, 2, 2), (0, 1.5, 0, 1, 0, 1), (0, 0, 0), 0.05, 4, True, False, 1.0),
     "halo_media_p4": (5, (4, 2, 2), (0, 1, 0, 1, 0, 1), (1, 0, 0), 0.0, 0, True, True, 1.0),
     "central_jitter_p2": (3, (3, 2, 2), (0, 1, 0, 1, 0, 1), (0, 1, 0), 0.05, 8, False, False, 0.0),
 }
@@ -101,11 +102,15 @@ def test_graph_replays_with_host_work_between():
     s.finalize()
 
 
-def test_tile_options_agree():
-    m, o, c0 = _build("media_pec_p5")
+@pytest.mark.parametrize("name", ["media_pec_p5", "periodic_jitter_p7", "media_pec_p7"])
+def test_tile_options_agree(name):
+    """Both tile options give the same bits: other elements per workgroup, and
+    at n1 = 8 in FP32 the matrix-core volume (one fused multiply-add per step,
+    in the vector path's order)."""
+    m, o, c0 = _build(name)
     st0 = random_state(o, 2)
     dt = o.auto_dt()
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp64", "fp32", "fp32_k64"):
         a = run_product(m, st0, dt, 3, c0, prec, "fused", tile_option=0)
         b = run_product(m, st0, dt, 3, c0, prec, "fused", tile_option=1)
         assert np.array_equal(a, b), prec
