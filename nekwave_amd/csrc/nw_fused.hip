@@ -279,7 +279,8 @@ void k_fused_stage(FusedArgs A) {
     //   two groups ahead: the face-neighbour table rows (they address the
     //   neighbour-trace gather) and the element records (staged into LDS one
     //   iteration ahead);
-    //   one group ahead: the fields and the neighbour traces;
+    //   one group ahead, after the iteration's first barrier: the fields and
+    //   the neighbour traces;
     //   at the top of the iteration: LSRK registers and flux weights, needed
     //   only at its end.
     struct Nbr {
@@ -411,7 +412,6 @@ void k_fused_stage(FusedArgs A) {
             }
         }
         Pre nxt;
-        fetch(gB, nb1, nxt);  // in flight while this group computes
 
         const ElemRec& ER = sE[t & 1][es];
 
@@ -440,6 +440,12 @@ void k_fused_stage(FusedArgs A) {
         for (int c = 0; c < 6; ++c) sU[es][c][m] = cur.u[c];
         __syncthreads();
         put_rec(rec1, (t + 1) & 1);  // read from the next iteration on, after the barrier below
+        // The next group's fields and neighbour traces: issued after the barrier
+        // so the burst of vector-memory instructions at the top of the
+        // iteration, which every wave of the CU issues at once, stays short
+        // and the barrier comes sooner; they still land before the iteration
+        // ends (and in flight while this group computes).
+        fetch(gB, nb1, nxt);
 
         // ---- 2. volume: reference derivatives (local_grad3, dg_kernels.cu:101-124)
         // folded straight into J * curl (dg_kernels.cu:140-149) ----
