@@ -212,7 +212,7 @@ def main():
     npts, nelt, dt = wl.npts, wl.nelt, wl.dt
     dof = 6 * npts
 
-    ms, wall, sinfo, _ = time_path(nw, wl, args.precision, "fused", args.steps, args.warmup)
+    ms, wall, sinfo, segs = time_path(nw, wl, args.precision, "fused", args.steps, args.warmup)
     value = dof * args.steps / (ms * 1e-3)
     s_u, s_k = SIZES[args.precision]
     b_step = algorithmic_bytes_per_step(npts, nelt, s_u, s_k)
@@ -272,6 +272,8 @@ def main():
                                "N=7 (n1=8), Gaussian Ez pulse, upwind, LSRK45 auto dt",
                    "nodes": npts, "dof": dof, "elements": nelt, "dt": dt, "pipeline": "fused",
                    "precision": args.precision, "elements_per_workgroup": sinfo["epw"],
+                   "tile_option": segs[0]["option"],
+                   "volume_contractions": "matrix cores" if segs[0]["matrix_volume"] else "vector ALUs",
                    "device": dev["arch"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,

@@ -177,6 +177,10 @@ int nw_dg_diag_instance(nw_dg* s, int instance, nw_diag* out);
 int nw_dg_tune(nw_dg* s);
 /* n1, element count and chosen elements per workgroup of segment `seg`. */
 int nw_dg_segment_info(const nw_dg* s, int seg, int* n1, int* nelt, int* epw);
+/* Extension: the tile option segment `seg` runs (0/1, as tuned or set) and
+ * whether that option computes the volume contractions on the matrix cores
+ * (n1 = 8 with FP32 fields, option 1; results equal the vector path's). */
+int nw_dg_segment_option(const nw_dg* s, int seg, int* option, int* matrix_volume);
 /* DgSolver::finalize (dg_solver.cpp:529-582) + destructor. */
 void nw_dg_destroy(nw_dg* s);
 

@@ -296,6 +296,23 @@ class DgSolver:
             raise NekWaveError(_lib.NW_ERR_STATE, "solver not initialized")
         return self._h
 
+    def tune(self):
+        """Time both tile options per segment and keep the faster (nw_dg_tune)."""
+        check(lib().nw_dg_tune(self._handle()))
+
+    def segments(self):
+        """Per n1 segment: n1, elements, elements per workgroup, tile option and
+        whether it runs the volume contractions on the matrix cores."""
+        out = []
+        for s in range(self.info()["segments"]):
+            n1, nelt, epw = C.c_int(), C.c_int(), C.c_int()
+            check(lib().nw_dg_segment_info(self._handle(), s, C.byref(n1), C.byref(nelt), C.byref(epw)))
+            opt, mx = C.c_int(), C.c_int()
+            check(lib().nw_dg_segment_option(self._handle(), s, C.byref(opt), C.byref(mx)))
+            out.append({"n1": n1.value, "nelt": nelt.value, "epw": epw.value, "option": opt.value,
+                        "matrix_volume": bool(mx.value)})
+        return out
+
     def __del__(self):
         try:
             self.finalize()
@@ -377,17 +394,6 @@ class BatchSolver(DgSolver):
         o = _lib.Diag()
         check(lib().nw_dg_diag_instance(self._handle(), int(instance), C.byref(o)))
         return {"energy": o.energy, "max_e": o.max_e, "max_h": o.max_h}
-
-    def tune(self):
-        check(lib().nw_dg_tune(self._handle()))
-
-    def segments(self):
-        out = []
-        for s in range(self.info()["segments"]):
-            n1, nelt, epw = C.c_int(), C.c_int(), C.c_int()
-            check(lib().nw_dg_segment_info(self._handle(), s, C.byref(n1), C.byref(nelt), C.byref(epw)))
-            out.append({"n1": n1.value, "nelt": nelt.value, "epw": epw.value})
-        return out
 
 
 from .case import Case, Config  # noqa: E402

@@ -281,6 +281,14 @@ NW_API int nw_dg_segment_info(const nw_dg* s, int seg, int* n1, int* nelt, int* 
     if (epw) *epw = nw::fused_epw(g.n1, g.option, s->s.prec);
     return NW_OK;
 }
+NW_API int nw_dg_segment_option(const nw_dg* s, int seg, int* option, int* matrix_volume) {
+    NW_CHECK_ARG(s && seg >= 0 && seg < (int)s->s.seg.size(), "bad segment");
+    const nw::Segment& g = s->s.seg[seg];
+    const bool fused = s->s.pipe == NW_PIPE_FUSED;
+    if (option) *option = g.option;
+    if (matrix_volume) *matrix_volume = fused && nw::fused_matrix_volume(g.n1, g.option, s->s.prec) ? 1 : 0;
+    return NW_OK;
+}
 NW_API int nw_dg_step(nw_dg* s, double dt, double time) {
     (void)time;
     NW_CHECK_ARG(s, "null solver");

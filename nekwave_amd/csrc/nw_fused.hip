@@ -718,6 +718,10 @@ int fused_epw(int n1, int option, int prec) {
     return epw_of(n1, option ? 1 : 0, prec == kFP64 ? 8 : 4);
 }
 
+bool fused_matrix_volume(int n1, int option, int prec) {
+    return matrix_volume(n1, option ? 1 : 0, prec == kFP64 ? 8 : 4);
+}
+
 bool fused_supported(int n1, int prec) { return n1 >= 2 && n1 <= 9 && prec >= 0 && prec < kNumPrec; }
 
 hipError_t launch_fused(int n1, int prec, int option, const FusedArgs& a, hipStream_t s) {

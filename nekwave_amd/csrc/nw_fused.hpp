@@ -38,6 +38,8 @@ struct FusedArgs {
 
 // Elements per workgroup options (index 0/1) for a given n1.
 int fused_epw(int n1, int option, int prec);
+// Whether tile option `option` computes the volume contractions on the matrix cores.
+bool fused_matrix_volume(int n1, int option, int prec);
 bool fused_supported(int n1, int prec);
 hipError_t launch_fused(int n1, int prec, int option, const FusedArgs& a, hipStream_t s);
 int fused_upload_tables(const GllTables& t);

@@ -116,6 +116,16 @@ def test_tile_options_agree(name):
         assert np.array_equal(a, b), prec
 
 
+def test_segment_option_reports_matrix_volume():
+    m, o, c0 = _build("periodic_jitter_p7")
+    for opt, prec, want in [(0, "fp32", False), (1, "fp32", True), (1, "fp32_k64", True), (1, "fp64", False)]:
+        s = nw.DgSolver(precision=prec, pipeline="fused", tile_option=opt)
+        s.initialize(m, c0)
+        seg = s.segments()[0]
+        s.finalize()
+        assert (seg["n1"], seg["option"], seg["matrix_volume"]) == (8, opt, want), (prec, seg)
+
+
 @pytest.mark.parametrize("pipeline,precision", [("plain", "fp64"), ("fused", "fp64"), ("fused", "fp32")])
 def test_device_diagnostics_match_oracle(pipeline, precision):
     m, o, c0 = _build("media_pec_p5")
