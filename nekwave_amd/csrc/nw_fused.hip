@@ -9,11 +9,13 @@
 // workgroup.  u ping-pongs (neighbours read the pre-stage trace of u_in);
 // k is element-local and updated in place.
 //
-// Work layout: a workgroup owns EPW elements, one thread per node.  Every HBM
-// read of the stage -- the node's six field values, its six LSRK registers and
-// the neighbour trace of the thread's face point -- is issued before the first
-// barrier, so a wave has all its memory in flight at once.  The element's
-// fields then sit in LDS for the three tensor contractions.  Face point
+// Work layout: a persistent workgroup walks element groups of EPW elements,
+// one thread per node.  The HBM reads of a group -- the node's six field
+// values and the neighbour trace of the thread's face point -- are issued one
+// iteration ahead, while the previous group computes; its LSRK registers at
+// the top of its own iteration.  The element's fields sit in LDS for the
+// three tensor contractions (vector ALUs, or at n1 = 8 optionally the matrix
+// cores, with identical results).  Face point
 // (f, p, q) is computed by one thread (face points outnumber nodes only for
 // n1 < 6, where a thread takes two); the terms dA*F go through LDS and each
 // node gathers them in fixed face order 0..5, so the lift is deterministic and
@@ -193,11 +195,13 @@ __device__ __forceinline__ void norm_and_inverse(double n2, double& len, double&
 // compute.  Per group:
 //   1. the node's face (Nanson) vectors from its FP64 geometry (computed
 //      during the previous iteration) and its field values go to LDS; barrier
-//   2. volume: the three reference derivatives of each field from LDS folded
-//      straight into J * curl;
+//   2. the next group's reads are issued; volume: the three reference
+//      derivatives of each field from LDS folded straight into J * curl
+//      (matrix-core variant: derivatives to LDS, folded after the barrier);
 //   3. face point (f, p, q): jumps, flux, dA -> LDS;                barrier
 //   4. lift: each node gathers its (at most three) face terms, one per
-//      direction, in fixed face order; inverse mass; LSRK update; stores.
+//      direction, in fixed face order; inverse mass; LSRK update; stores;
+//      the next group's geometry.
 template <int N1, int EPW, bool FIRST, typename R, typename KR, bool MX>
 __global__ __launch_bounds__(EPW* N1* N1* N1) __attribute__((amdgpu_waves_per_eu(WavesPerEU<EPW * N1 * N1 * N1, sizeof(R)>::value)))
 void k_fused_stage(FusedArgs A) {
