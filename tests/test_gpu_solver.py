@@ -177,6 +177,48 @@ def test_tm110_c1_on_device(pipeline, precision):
     assert err < 2e-3, err
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_boxper_mode_on_device(precision):
+    """3dboxper_test restated on the device: [0, 2 pi]^3 periodic 3x3x3, order 4,
+    central flux, auto dt, 50 steps, relative L2 to the analytic mode <= 0.05;
+    and within 2e-5 of the oracle's own error."""
+    L = 2 * math.pi
+    m, o = pair(5, (3, 3, 3), (0, L, 0, L, 0, L), (1, 1, 1))
+    dt = o.auto_dt(1.0, 0.4)
+    st0 = an.pack(an.boxper_mode(o.x, o.y, o.zc, 0.0))
+    ex = an.pack(an.boxper_mode(o.x, o.y, o.zc, 50 * dt))
+    got = run_product(m, st0, dt, 50, 0.0, precision, "fused")
+    err = an.rel_l2(an.dV(o), got, ex)
+    err_o = an.rel_l2(an.dV(o), run_oracle(o, st0, dt, 50, 0.0), ex)
+    print("boxper %s relL2=%.3e (oracle %.3e)" % (precision, err, err_o))
+    assert err <= 0.05 and abs(err - err_o) <= 2e-5, (err, err_o)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_p_convergence_on_device(precision):
+    """TM110 on a 2^3 PEC cube to T = 0.3: each order cuts the error by more
+    than 3 (spectral convergence; oracle errors 8.7e-2, 8.6e-3, 6.2e-4, 3.5e-5,
+    1.7e-6 for n1 = 3..7), and each error equals the oracle's to 1e-10 (FP64)
+    or 1e-6 (FP32, which stops at n1 = 6, above its rounding floor)."""
+    T = 0.3
+    errs = []
+    tol = 1e-10 if precision == "fp64" else 1e-6
+    for n1 in ((3, 4, 5, 6, 7) if precision == "fp64" else (3, 4, 5, 6)):
+        m, o = pair(n1, (2, 2, 2))
+        nst = int(math.ceil(T / (0.2 * o.dxmin())))
+        dt = T / nst
+        st0 = an.pack(an.tm_mode(o.x, o.y, 0.0))
+        ex = an.pack(an.tm_mode(o.x, o.y, T))
+        got = run_product(m, st0, dt, nst, 1.0, precision, "fused")
+        e = an.rel_l2(an.dV(o), got, ex)
+        e_o = an.rel_l2(an.dV(o), run_oracle(o, st0, dt, nst, 1.0), ex)
+        assert abs(e - e_o) <= tol, (n1, e, e_o)
+        errs.append(e)
+    print("p-convergence %s" % precision, errs)
+    for e0, e1 in zip(errs, errs[1:]):
+        assert e1 < e0 / 3, errs
+
+
 def test_case_lifecycle_boxpec_mode():
     """3dboxpec_test.cpp restated: Case with a 3x3x3 [-1,1]^3 box, order 4,
     central flux, auto dt, 50 steps, relative L2 <= 0.05."""
