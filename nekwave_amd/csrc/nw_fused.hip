@@ -219,7 +219,8 @@ void k_fused_stage(FusedArgs A) {
     static_assert(!MX || (N1 == 8 && EPW == 1 && sizeof(R) == 4), "matrix-core volume shape");
     __shared__ R sU[EPW][6][NP];
     __shared__ R sF[EPW * NF + 1][6];  // face terms dA*F; the last row stays zero
-    __shared__ R sC[EPW * NF + 64][9];  // per face point: Nanson vector, minus-side trace (6); 64 spare rows
+    constexpr int SPARE = sizeof(R) == 4 ? 64 : 0;  // spare face rows (branch-free writes, FP32)
+    __shared__ R sC[EPW * NF + SPARE][9];  // per face point: Nanson vector, minus-side trace (6)
     __shared__ R sD[N2];               // D, row-major: sD[i * N1 + l] = D_il
     __shared__ ElemRec sE[2][EPW];     // element records, double-buffered over groups
     // matrix-core path: the 18 reference derivatives of every node, by direction
@@ -428,25 +429,34 @@ void k_fused_stage(FusedArgs A) {
         // The node also hands its field values (the face point's minus-side
         // trace) to the face point: rows of 9 values are read without LDS bank
         // conflicts, which strided reads of the face nodes out of sU are not.
-        // Branch-free: every node writes one row per direction -- its face
-        // point's row where it lies on that direction's face, else a spare
-        // row of its lane that nobody reads (no branch around the LDS stores,
-        // so all waves do the same work).
-        {
+        // FP32: branch-free, every node writes one row per direction -- its
+        // face point's row where it lies on that direction's face, else a
+        // spare row of its lane that nobody reads (no branches around the LDS
+        // stores; measured faster).  FP64: only the face nodes write (the
+        // spare rows would double the LDS bytes; measured slower).
+        auto row = [&](int x, R a, R b, R c) {
+            R* o = sC[x];
+            o[0] = a; o[1] = b; o[2] = c;
+#pragma unroll
+            for (int c2 = 0; c2 < 6; ++c2) o[3 + c2] = cur.u[c2];
+        };
+        if constexpr (SPARE > 0) {
             const int spare = EPW * NF + (tid & 63);
             const int rs = j == 0 ? es * NF + 0 * N2 + i + N1 * k : (j == N1 - 1 ? es * NF + 2 * N2 + i + N1 * k : spare);
             const int rr = i == N1 - 1 ? es * NF + 1 * N2 + j + N1 * k : (i == 0 ? es * NF + 3 * N2 + j + N1 * k : spare);
             const int rt = k == 0 ? es * NF + 4 * N2 + i + N1 * j : (k == N1 - 1 ? es * NF + 5 * N2 + i + N1 * j : spare);
             const R ss = j == 0 ? R(-1) : R(1), sr = i == 0 ? R(-1) : R(1), st = k == 0 ? R(-1) : R(1);
-            auto row = [&](int x, R a, R b, R c) {
-                R* o = sC[x];
-                o[0] = a; o[1] = b; o[2] = c;
-#pragma unroll
-                for (int c2 = 0; c2 < 6; ++c2) o[3 + c2] = cur.u[c2];
-            };
             row(rs, ss * Csx, ss * Csy, ss * Csz);
             row(rr, sr * Crx, sr * Cry, sr * Crz);
             row(rt, st * Ctx, st * Cty, st * Ctz);
+        } else {
+            const int b = es * NF;
+            if (j == 0) row(b + 0 * N2 + i + N1 * k, -Csx, -Csy, -Csz);
+            if (i == N1 - 1) row(b + 1 * N2 + j + N1 * k, Crx, Cry, Crz);
+            if (j == N1 - 1) row(b + 2 * N2 + i + N1 * k, Csx, Csy, Csz);
+            if (i == 0) row(b + 3 * N2 + j + N1 * k, -Crx, -Cry, -Crz);
+            if (k == 0) row(b + 4 * N2 + i + N1 * j, -Ctx, -Cty, -Ctz);
+            if (k == N1 - 1) row(b + 5 * N2 + i + N1 * j, Ctx, Cty, Ctz);
         }
 #pragma unroll
         for (int c = 0; c < 6; ++c) sU[es][c][m] = cur.u[c];
