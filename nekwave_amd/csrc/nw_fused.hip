@@ -223,6 +223,13 @@ void k_fused_stage(FusedArgs A) {
     __shared__ R sC[EPW * NF + SPARE][9];  // per face point: Nanson vector, minus-side trace (6)
     __shared__ R sD[N2];               // D, row-major: sD[i * N1 + l] = D_il
     __shared__ ElemRec sE[2][EPW];     // element records, double-buffered over groups
+    // n1 = 8, FP32, vector-ALU volume: the fields again with the s-lines (i, k)
+    // and t-lines (i, j) contiguous, lines 12 words apart (16-byte aligned, and
+    // eight lines cover all 32 banks), so every direction reads its line with
+    // 16-byte accesses instead of 4-byte ones at strides of 8 and 64 words
+    constexpr bool TL = N1 == 8 && EPW == 1 && sizeof(R) == 4 && !MX;
+    __shared__ __attribute__((aligned(16))) R sUs[TL ? 6 : 1][TL ? 12 * N2 : 1];
+    __shared__ __attribute__((aligned(16))) R sUt[TL ? 6 : 1][TL ? 12 * N2 : 1];
     // matrix-core path: the 18 reference derivatives of every node, by direction
     // r (node order), s (line (i, k) major, j fastest), t (line (i, j) major, k
     // fastest, lines 9 words apart so the node-order reads are conflict-free)
@@ -460,6 +467,13 @@ void k_fused_stage(FusedArgs A) {
         }
 #pragma unroll
         for (int c = 0; c < 6; ++c) sU[es][c][m] = cur.u[c];
+        if constexpr (TL) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                sUs[c][(i + 8 * k) * 12 + j] = cur.u[c];
+                sUt[c][(i + 8 * j) * 12 + k] = cur.u[c];
+            }
+        }
         __syncthreads();
         put_rec(rec1, (t + 1) & 1);  // read from the next iteration on, after the barrier below
         // The next group's fields and neighbour traces: issued after the barrier
@@ -535,7 +549,11 @@ void k_fused_stage(FusedArgs A) {
                 for (int q = 0; q < N1; ++q) {
                     const int at = d == 0 ? q + N1 * j + N2 * k : (d == 1 ? i + N1 * q + N2 * k : i + N1 * j + N2 * q);
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) a[c] += Dr[q] * sU[es][3 * h + c][at];
+                    for (int c = 0; c < 3; ++c) {
+                        const R v = !TL || d == 0 ? sU[es][3 * h + c][at]
+                                                  : (d == 1 ? sUs[3 * h + c][(i + 8 * k) * 12 + q] : sUt[3 * h + c][(i + 8 * j) * 12 + q]);
+                        a[c] += Dr[q] * v;
+                    }
                 }
                 // J d/dx_e of component c gets a * C_d,e;
                 // curl u = (dy u_z - dz u_y, dz u_x - dx u_z, dx u_y - dy u_x)
