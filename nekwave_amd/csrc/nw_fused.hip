@@ -223,13 +223,15 @@ void k_fused_stage(FusedArgs A) {
     __shared__ R sC[EPW * NF + SPARE][9];  // per face point: Nanson vector, minus-side trace (6)
     __shared__ R sD[N2];               // D, row-major: sD[i * N1 + l] = D_il
     __shared__ ElemRec sE[2][EPW];     // element records, double-buffered over groups
-    // n1 = 8, FP32, vector-ALU volume: the fields again with the s-lines (i, k)
-    // and t-lines (i, j) contiguous, lines 12 words apart (16-byte aligned, and
-    // eight lines cover all 32 banks), so every direction reads its line with
-    // 16-byte accesses instead of 4-byte ones at strides of 8 and 64 words
-    constexpr bool TL = N1 == 8 && EPW == 1 && sizeof(R) == 4 && !MX;
-    __shared__ __attribute__((aligned(16))) R sUs[TL ? 6 : 1][TL ? 12 * N2 : 1];
-    __shared__ __attribute__((aligned(16))) R sUt[TL ? 6 : 1][TL ? 12 * N2 : 1];
+    // n1 = 8, 9, FP32, vector-ALU volume: the fields again with the s-lines
+    // (i, k) and t-lines (i, j) contiguous, lines 12 words apart (16-byte
+    // aligned; at n1 = 8 eight lines cover all 32 banks), so those directions
+    // read their lines with 16-byte accesses instead of 4-byte ones at strides
+    // of n1 and n1^2 words (measured faster at n1 = 8, 9; slower at 6 and 7)
+    constexpr bool TL = (N1 == 8 || N1 == 9) && EPW == 1 && sizeof(R) == 4 && !MX;
+    constexpr int RW = 12;
+    __shared__ __attribute__((aligned(16))) R sUs[TL ? 6 : 1][TL ? RW * N2 : 1];
+    __shared__ __attribute__((aligned(16))) R sUt[TL ? 6 : 1][TL ? RW * N2 : 1];
     // matrix-core path: the 18 reference derivatives of every node, by direction
     // r (node order), s (line (i, k) major, j fastest), t (line (i, j) major, k
     // fastest, lines 9 words apart so the node-order reads are conflict-free)
@@ -470,8 +472,8 @@ void k_fused_stage(FusedArgs A) {
         if constexpr (TL) {
 #pragma unroll
             for (int c = 0; c < 6; ++c) {
-                sUs[c][(i + 8 * k) * 12 + j] = cur.u[c];
-                sUt[c][(i + 8 * j) * 12 + k] = cur.u[c];
+                sUs[c][(i + N1 * k) * RW + j] = cur.u[c];
+                sUt[c][(i + N1 * j) * RW + k] = cur.u[c];
             }
         }
         __syncthreads();
@@ -551,7 +553,7 @@ void k_fused_stage(FusedArgs A) {
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
                         const R v = !TL || d == 0 ? sU[es][3 * h + c][at]
-                                                  : (d == 1 ? sUs[3 * h + c][(i + 8 * k) * 12 + q] : sUt[3 * h + c][(i + 8 * j) * 12 + q]);
+                                                  : (d == 1 ? sUs[3 * h + c][(i + N1 * k) * RW + q] : sUt[3 * h + c][(i + N1 * j) * RW + q]);
                         a[c] += Dr[q] * v;
                     }
                 }
