@@ -130,7 +130,8 @@ typedef struct nw_opts {
     int pipeline;     /* NW_PIPE_* (default FUSED) */
     int use_graph;    /* capture each step as a hipGraph (default 1) */
     int device;       /* HIP device ordinal (default 0) */
-    int tile_option;  /* fused elements-per-workgroup option 0/1, -1 = tuned default */
+    int tile_option;  /* fused tile option 0/1 (elements per workgroup; at n1 = 8 in FP32,
+                       * 1 = matrix-core volume), -1 = tuned default */
 } nw_opts;
 void nw_opts_default(nw_opts* o);
 
@@ -172,8 +173,8 @@ int nw_batch_create(int count, const nw_mesh* const* meshes, double c0, const nw
 int nw_dg_upload_instance(nw_dg* s, int instance, const double* host, size_t n);
 int nw_dg_download_instance(nw_dg* s, int instance, double* host, size_t n);
 int nw_dg_diag_instance(nw_dg* s, int instance, nw_diag* out);
-/* Autotuner: time each elements-per-workgroup option per segment and keep the
- * fastest (runs by itself before the first step when tile_option = -1). */
+/* Autotuner: time both tile options per segment and keep the faster (runs by
+ * itself before the first step when tile_option = -1). */
 int nw_dg_tune(nw_dg* s);
 /* n1, element count and chosen elements per workgroup of segment `seg`. */
 int nw_dg_segment_info(const nw_dg* s, int seg, int* n1, int* nelt, int* epw);
