@@ -128,12 +128,12 @@ def load_traffic(name):
     return None
 
 
-def profile_run(nw, W, precision, steps):
+def profile_run(nw, W, precision, steps, workload="c2"):
     import ctypes
 
-    wl = W.c2()
+    wl = getattr(W, workload)()
     s = nw.DgSolver(precision=precision, pipeline="fused", use_graph=False, tile_option=0)
-    s.initialize(wl.meshes[0], 1.0)
+    s.initialize(wl.meshes[0], wl.c0)
     s.uploadState(wl.states[0])
     s.run(wl.dt, steps)
     s.finalize()
@@ -190,6 +190,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--accuracy-steps", type=int, default=1000)
     ap.add_argument("--configs", default="c3,c4,c5", help="other workloads to time (comma list, or 'none')")
+    ap.add_argument("--profile-workload", default="c2", choices=["c2", "c3", "c5"],
+                    help="workload of --profile-run (single-mesh workloads)")
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for rocprofv3 counter passes: the fused stage on the workload "
                          "plus a 256 MiB device copy as the HBM byte-count calibration")
@@ -203,7 +205,7 @@ def main():
     from nekwave_amd import workloads as W
 
     if args.profile_run:
-        profile_run(nw, W, args.precision, steps=min(args.steps, 4))
+        profile_run(nw, W, args.precision, steps=min(args.steps, 4), workload=args.profile_workload)
         return
 
     dev = nw.device_info(0)
