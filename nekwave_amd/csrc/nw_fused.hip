@@ -108,9 +108,6 @@ template <typename T>
 __device__ __forceinline__ T bld(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
     return bload(static_cast<T*>(nullptr), r, vo, so);
 }
-__device__ __forceinline__ int bld_i32(__amdgpu_buffer_rsrc_t r, unsigned vo) {
-    return static_cast<int>(__builtin_amdgcn_raw_buffer_load_b32(r, vo, 0, 0));
-}
 __device__ __forceinline__ int4 bld_int4(__amdgpu_buffer_rsrc_t r, unsigned vo) {
     typedef __attribute__((ext_vector_type(4))) int i32x4;
     const i32x4 v = __builtin_bit_cast(i32x4, __builtin_amdgcn_raw_buffer_load_b128(r, vo, 0, 0));
