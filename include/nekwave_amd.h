@@ -173,6 +173,12 @@ int nw_batch_create(int count, const nw_mesh* const* meshes, double c0, const nw
 int nw_dg_upload_instance(nw_dg* s, int instance, const double* host, size_t n);
 int nw_dg_download_instance(nw_dg* s, int instance, double* host, size_t n);
 int nw_dg_diag_instance(nw_dg* s, int instance, nw_diag* out);
+/* Extension: relative L2 distance of an instance's current state to a host
+ * reference state in the uploadState layout (6 component blocks of npts,
+ * FP64): sqrt(sum (u - ref)^2 dV / sum ref^2 dV), dV = J w_i w_j w_k -- the
+ * error norm of 3dboxpec_test.cpp:76-119 -- reduced on the device in FP64,
+ * without downloading the state. */
+int nw_dg_rel_l2(nw_dg* s, int instance, const double* ref, size_t n, double* rel_l2);
 /* Autotuner: time both tile options per segment and keep the faster (runs by
  * itself before the first step when tile_option = -1). */
 int nw_dg_tune(nw_dg* s);

@@ -153,6 +153,13 @@ class DgSolver {
         check(nw_dg_diag(s_, &d));
         return d;
     }
+    // Extension: relative L2 distance of the state to a host reference state
+    // (uploadState layout), reduced on the device (nw_dg_rel_l2).
+    double relL2(const double* ref, size_t n) const {
+        double r = 0.0;
+        check(nw_dg_rel_l2(s_, 0, ref, n, &r));
+        return r;
+    }
     nw_opts& options() { return opts_; }
 
    private:

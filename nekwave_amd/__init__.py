@@ -281,6 +281,15 @@ class DgSolver:
         check(lib().nw_dg_diag(self._handle(), C.byref(o)))
         return {"energy": o.energy, "max_e": o.max_e, "max_h": o.max_h}
 
+    def relL2(self, reference, instance=0):
+        """Relative L2 distance of the current state to `reference` (uploadState
+        layout), weighted by J w_i w_j w_k as 3dboxpec_test.cpp:76-119, reduced
+        on the device in FP64 (nw_dg_rel_l2)."""
+        a = np.ascontiguousarray(reference, dtype=np.float64).reshape(-1)
+        out = C.c_double()
+        check(lib().nw_dg_rel_l2(self._handle(), int(instance), _dp(a), a.size, C.byref(out)))
+        return out.value
+
     def info(self):
         o = _lib.DgInfo()
         check(lib().nw_dg_get_info(self._handle(), C.byref(o)))

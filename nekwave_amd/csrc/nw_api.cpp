@@ -261,6 +261,10 @@ NW_API int nw_dg_download_instance(nw_dg* s, int instance, double* host, size_t 
     NW_CHECK_ARG(s, "null solver");
     return s->s.download(instance, host, n);
 }
+NW_API int nw_dg_rel_l2(nw_dg* s, int instance, const double* ref, size_t n, double* rel_l2) {
+    NW_CHECK_ARG(s, "null solver");
+    return s->s.rel_l2(instance, ref, n, rel_l2);
+}
 NW_API int nw_dg_diag_instance(nw_dg* s, int instance, nw_diag* out) {
     NW_CHECK_ARG(s && out, "null argument");
     return s->s.diag(instance, out);

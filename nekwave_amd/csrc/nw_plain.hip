@@ -315,38 +315,47 @@ __global__ void k_face_data(const int* __restrict__ fcode, const int* __restrict
 // ---------------------------------------------------------------------------
 // diagnostics: energy (FP64), max |E|, max |H|; two-pass deterministic reduce
 // ---------------------------------------------------------------------------
+// Quadrature weight of node p: J w_i w_j w_k, J from the metric array when
+// given, else from the element's trilinear map (mesh.cpp:291-295).
+__device__ __forceinline__ double node_dv(long long p, int n1, const double* __restrict__ jac,
+                                          const ElemGeom* __restrict__ geom) {
+    const int n2 = n1 * n1, np = n2 * n1;
+    const long long e = p / np;
+    const int m = static_cast<int>(p - e * np);
+    const int i = m % n1, j = (m / n1) % n1, k = m / n2;
+    double J;
+    if (jac) {
+        J = jac[p];
+    } else {
+        const double r = c_gll.z[n1][i], s = c_gll.z[n1][j], t = c_gll.z[n1][k];
+        const double* c0 = geom[e].c[0];
+        const double* c1 = geom[e].c[1];
+        const double* c2 = geom[e].c[2];
+        const double xr = c0[1] + c0[4] * s + c0[5] * t + c0[7] * (s * t);
+        const double xs = c0[2] + c0[4] * r + c0[6] * t + c0[7] * (r * t);
+        const double xt = c0[3] + c0[5] * r + c0[6] * s + c0[7] * (r * s);
+        const double yr = c1[1] + c1[4] * s + c1[5] * t + c1[7] * (s * t);
+        const double ys = c1[2] + c1[4] * r + c1[6] * t + c1[7] * (r * t);
+        const double yt = c1[3] + c1[5] * r + c1[6] * s + c1[7] * (r * s);
+        const double zr = c2[1] + c2[4] * s + c2[5] * t + c2[7] * (s * t);
+        const double zs = c2[2] + c2[4] * r + c2[6] * t + c2[7] * (r * t);
+        const double zt = c2[3] + c2[5] * r + c2[6] * s + c2[7] * (r * s);
+        J = xr * (ys * zt - yt * zs) - xs * (yr * zt - yt * zr) + xt * (yr * zs - ys * zr);
+    }
+    return J * c_gll.w[n1][i] * c_gll.w[n1][j] * c_gll.w[n1][k];
+}
+
 template <typename R>
 __global__ void k_diag_partial(const R* __restrict__ u, long long cs, long long ns, long long npts, int n1,
                                const double* __restrict__ jac,
                                const ElemGeom* __restrict__ geom, const ElemMat* __restrict__ mat,
                                double* __restrict__ part) {
     __shared__ double sh[3][kBlock];
-    const int n2 = n1 * n1, np = n2 * n1;
+    const int np = n1 * n1 * n1;
     double en = 0.0, me = 0.0, mh = 0.0;
     for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < npts; p += (long long)gridDim.x * blockDim.x) {
         const long long e = p / np;
-        const int m = static_cast<int>(p - e * np);
-        const int i = m % n1, j = (m / n1) % n1, k = m / n2;
-        double J;
-        if (jac) {
-            J = jac[p];
-        } else {
-            const double r = c_gll.z[n1][i], s = c_gll.z[n1][j], t = c_gll.z[n1][k];
-            const double* c0 = geom[e].c[0];
-            const double* c1 = geom[e].c[1];
-            const double* c2 = geom[e].c[2];
-            const double xr = c0[1] + c0[4] * s + c0[5] * t + c0[7] * (s * t);
-            const double xs = c0[2] + c0[4] * r + c0[6] * t + c0[7] * (r * t);
-            const double xt = c0[3] + c0[5] * r + c0[6] * s + c0[7] * (r * s);
-            const double yr = c1[1] + c1[4] * s + c1[5] * t + c1[7] * (s * t);
-            const double ys = c1[2] + c1[4] * r + c1[6] * t + c1[7] * (r * t);
-            const double yt = c1[3] + c1[5] * r + c1[6] * s + c1[7] * (r * s);
-            const double zr = c2[1] + c2[4] * s + c2[5] * t + c2[7] * (s * t);
-            const double zs = c2[2] + c2[4] * r + c2[6] * t + c2[7] * (r * t);
-            const double zt = c2[3] + c2[5] * r + c2[6] * s + c2[7] * (r * s);
-            J = xr * (ys * zt - yt * zs) - xs * (yr * zt - yt * zr) + xt * (yr * zs - ys * zr);
-        }
-        const double dV = J * c_gll.w[n1][i] * c_gll.w[n1][j] * c_gll.w[n1][k];
+        const double dV = node_dv(p, n1, jac, geom);
         const R* up = u + p * ns;
         const double ex = up[0], ey = up[cs], ez = up[2 * cs];
         const double hx = up[3 * cs], hy = up[4 * cs], hz = up[5 * cs];
@@ -378,6 +387,52 @@ __global__ void k_diag_final(const double* __restrict__ part, int n, double* __r
             en += part[3 * b]; me = fmax(me, part[3 * b + 1]); mh = fmax(mh, part[3 * b + 2]);
         }
         out[0] = en; out[1] = sqrt(me); out[2] = sqrt(mh);
+    }
+}
+
+// Relative L2 distance to a reference state (3dboxpec_test.cpp:76-119):
+// sum over components and nodes of (u - ref)^2 dV and ref^2 dV, in FP64.
+// ref is in the reference's host layout (component blocks of npts).
+template <typename R>
+__global__ void k_l2_partial(const R* __restrict__ u, long long cs, long long ns, const double* __restrict__ ref,
+                             long long npts, int n1, const double* __restrict__ jac,
+                             const ElemGeom* __restrict__ geom, double* __restrict__ part) {
+    __shared__ double sh[2][kBlock];
+    double num = 0.0, den = 0.0;
+    for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < npts; p += (long long)gridDim.x * blockDim.x) {
+        const double dV = node_dv(p, n1, jac, geom);
+        const R* up = u + p * ns;
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            const double r = ref[c * npts + p], d = double(up[c * cs]) - r;
+            a += d * d;
+            b += r * r;
+        }
+        num += a * dV;
+        den += b * dV;
+    }
+    sh[0][threadIdx.x] = num; sh[1][threadIdx.x] = den;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sh[0][0]; part[2 * blockIdx.x + 1] = sh[1][0];
+    }
+}
+
+__global__ void k_l2_final(const double* __restrict__ part, int n, double* __restrict__ out) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        double num = 0.0, den = 0.0;
+        for (int b = 0; b < n; ++b) {
+            num += part[2 * b]; den += part[2 * b + 1];
+        }
+        out[0] = num; out[1] = den;
     }
 }
 
@@ -519,6 +574,18 @@ hipError_t launch_diag(const R* u, long long cs, long long ns, long long npts, i
     hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s, part, nb, out);
     return hipGetLastError();
 }
+template <typename R>
+hipError_t launch_l2(const R* u, long long cs, long long ns, const double* ref, long long npts, int n1, const double* jac,
+                     const ElemGeom* geom, double* part, double* out, hipStream_t s) {
+    const int nb = diag_blocks();
+    hipLaunchKernelGGL(k_l2_partial<R>, dim3(nb), dim3(kBlock), 0, s, u, cs, ns, ref, npts, n1, jac, geom, part);
+    hipLaunchKernelGGL(k_l2_final, dim3(1), dim3(64), 0, s, part, nb, out);
+    return hipGetLastError();
+}
+template hipError_t launch_l2<double>(const double*, long long, long long, const double*, long long, int, const double*,
+                                      const ElemGeom*, double*, double*, hipStream_t);
+template hipError_t launch_l2<float>(const float*, long long, long long, const double*, long long, int, const double*,
+                                     const ElemGeom*, double*, double*, hipStream_t);
 template hipError_t launch_diag<double>(const double*, long long, long long, long long, int, const double*,
                                         const ElemGeom*, const ElemMat*, double*, double*, hipStream_t);
 template hipError_t launch_diag<float>(const float*, long long, long long, long long, int, const double*, const ElemGeom*,

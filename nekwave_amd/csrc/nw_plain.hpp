@@ -46,6 +46,11 @@ int diag_blocks();
 template <typename R>
 hipError_t launch_diag(const R* u, long long cs, long long ns, long long npts, int n1, const double* jac,
                        const ElemGeom* geom, const ElemMat* mat, double* part, double* out, hipStream_t s);
+// Sums of (u - ref)^2 dV and ref^2 dV over all components and nodes into
+// out[0], out[1] (FP64; ref in component blocks of npts).
+template <typename R>
+hipError_t launch_l2(const R* u, long long cs, long long ns, const double* ref, long long npts, int n1, const double* jac,
+                     const ElemGeom* geom, double* part, double* out, hipStream_t s);
 template <typename D, typename S>
 hipError_t launch_convert(D* dst, long long dcs, long long dns, const S* src, long long scs, long long sns, long long n,
                           int ncomp, hipStream_t s);

@@ -4,6 +4,7 @@
 #include "nw_solver.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <mutex>
@@ -632,6 +633,34 @@ int Solver::diag(int ix, nw_diag* out) {
     out->energy = h[0];
     out->max_e = h[1];
     out->max_h = h[2];
+    return NW_OK;
+}
+
+// Relative L2 distance of instance ix's state to a host reference state (the
+// reference's layout, 6 component blocks of npts), weighted by J w_i w_j w_k
+// (3dboxpec_test.cpp:76-119), reduced on the device in FP64.
+int Solver::rel_l2(int ix, const double* ref, size_t n, double* out) {
+    if (!initialized) { set_error("rel_l2 before create"); return NW_ERR_STATE; }
+    NW_CHECK_ARG(ix >= 0 && ix < (int)inst.size(), "bad instance index");
+    const Instance& I = inst[ix];
+    NW_CHECK_ARG(ref != nullptr && out != nullptr && n == size_t(6 * I.npts), "rel_l2: reference size must be 6*npts");
+    NW_TRY_HIP(hipSetDevice(device));
+    NW_TRY_HIP(hipMemcpyAsync(d_stage, ref, 8 * 6 * I.npts, hipMemcpyHostToDevice, stream));
+    const ElemGeom* geom = d_geom + I.elem_base;
+    if (pipe == NW_PIPE_PLAIN) {
+        NW_TRY_HIP(launch_l2<double>(d_state, stride, 1, d_stage, npts, n1, d_met[3], nullptr, d_diag_part, d_diag_out,
+                                     stream));
+    } else if (prec == kFP64) {
+        NW_TRY_HIP(launch_l2<double>(static_cast<const double*>(d_u[cur]) + 6 * I.node_base, 1, 6, d_stage, I.npts,
+                                     I.n1, nullptr, geom, d_diag_part, d_diag_out, stream));
+    } else {
+        NW_TRY_HIP(launch_l2<float>(static_cast<const float*>(d_u[cur]) + 6 * I.node_base, 1, 6, d_stage, I.npts, I.n1,
+                                    nullptr, geom, d_diag_part, d_diag_out, stream));
+    }
+    double h[2];
+    NW_TRY_HIP(hipMemcpyAsync(h, d_diag_out, sizeof(h), hipMemcpyDeviceToHost, stream));
+    NW_TRY_HIP(hipStreamSynchronize(stream));
+    *out = h[1] > 0.0 ? std::sqrt(h[0] / h[1]) : (h[0] > 0.0 ? HUGE_VAL : 0.0);
     return NW_OK;
 }
 

@@ -73,6 +73,7 @@ struct Solver {
     int launch_steps(double dt, int nsteps);
     int time(double dt, int nsteps, double* ms);
     int diag(int instance, nw_diag* out);
+    int rel_l2(int instance, const double* ref, size_t n, double* out);
     int tune();
 
    private:

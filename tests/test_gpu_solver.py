@@ -126,6 +126,30 @@ def test_segment_option_reports_matrix_volume():
         assert (seg["n1"], seg["option"], seg["matrix_volume"]) == (8, opt, want), (prec, seg)
 
 
+@pytest.mark.parametrize("name", ["media_pec_p5", "periodic_jitter_p7", "halo_media_p4"])
+@pytest.mark.parametrize("pipeline,precision", [("plain", "fp64"), ("fused", "fp64"), ("fused", "fp32")])
+def test_device_rel_l2_matches_host(name, pipeline, precision):
+    """nw_dg_rel_l2 (3dboxpec_test.cpp:76-119 norm, reduced on the device) equals
+    the host formula on the downloaded state, and is 0 against the state itself."""
+    if pipeline == "plain" and name == "halo_media_p4":
+        pytest.skip("covered by the fused pipeline")
+    m, o, c0 = _build(name)
+    st0 = random_state(o, 7)
+    dt = 0.5 * o.auto_dt()
+    s = nw.DgSolver(precision=precision, pipeline=pipeline)
+    s.initialize(m, c0)
+    s.uploadState(st0)
+    s.run(dt, 3)
+    got = s.downloadState()
+    ref = run_oracle(o, st0, dt, 4, c0)
+    host = rel_l2(o, got, ref)
+    dev = s.relL2(ref)
+    assert s.relL2(got) == 0.0
+    s.finalize()
+    print("%s %s %s device %.15e host %.15e" % (name, pipeline, precision, dev, host))
+    assert host > 0 and abs(dev - host) <= 1e-12 * host, (dev, host)
+
+
 @pytest.mark.parametrize("pipeline,precision", [("plain", "fp64"), ("fused", "fp64"), ("fused", "fp32")])
 def test_device_diagnostics_match_oracle(pipeline, precision):
     m, o, c0 = _build("media_pec_p5")
