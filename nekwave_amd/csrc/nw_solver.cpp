@@ -530,32 +530,34 @@ int Solver::build_graph(int parity) {
     return NW_OK;
 }
 
-// Autotuner: per n1 segment, time each elements-per-workgroup option on the
-// first-stage kernel (it writes u_out and k only, both rewritten by the next
-// real stage) and keep the fastest.  Runs at a step boundary, before graphs
-// are captured.
+// Autotuner: per n1 segment, time both tile options on a regular stage (it
+// reads k and writes u_out and k, all rewritten by the first real stage, which
+// does not read k) and keep the faster.  The options alternate twice and each
+// keeps its best time, so a slow moment of the GPU does not decide.  Runs at a
+// step boundary, before graphs are captured.
 int Solver::tune() {
     if (pipe != NW_PIPE_FUSED) return NW_OK;
     NW_TRY_HIP(hipSetDevice(device));
     for (Segment& sg : seg) {
-        float best = 1e30f;
-        int bestopt = 0;
-        for (int opt = 0; opt < 2; ++opt) {
-            int st = launch_segment(sg, 0, cur, opt, stream);  // warm
+        float best[2] = {1e30f, 1e30f};
+        for (int opt = 0; opt < 2; ++opt) {  // warm both (first launch sets up the kernel)
+            const int st = launch_segment(sg, 1, cur, opt, stream);
             if (st) return st;
-            NW_TRY_HIP(hipEventRecord(ev0, stream));
-            for (int r = 0; r < 3; ++r)
-                if ((st = launch_segment(sg, 0, cur, opt, stream))) return st;
-            NW_TRY_HIP(hipEventRecord(ev1, stream));
-            NW_TRY_HIP(hipEventSynchronize(ev1));
-            float ms = 0.0f;
-            NW_TRY_HIP(hipEventElapsedTime(&ms, ev0, ev1));
-            if (ms < best) {
-                best = ms;
-                bestopt = opt;
-            }
         }
-        sg.option = bestopt;
+        for (int rep = 0; rep < 2; ++rep)
+            for (int opt = 0; opt < 2; ++opt) {
+                NW_TRY_HIP(hipEventRecord(ev0, stream));
+                for (int r = 0; r < 3; ++r) {
+                    const int st = launch_segment(sg, 1, cur, opt, stream);
+                    if (st) return st;
+                }
+                NW_TRY_HIP(hipEventRecord(ev1, stream));
+                NW_TRY_HIP(hipEventSynchronize(ev1));
+                float ms = 0.0f;
+                NW_TRY_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+                best[opt] = std::min(best[opt], ms);
+            }
+        sg.option = best[1] < best[0] ? 1 : 0;
     }
     tuned = true;
     return NW_OK;
