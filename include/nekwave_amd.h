@@ -179,6 +179,24 @@ int nw_dg_diag_instance(nw_dg* s, int instance, nw_diag* out);
  * error norm of 3dboxpec_test.cpp:76-119 -- reduced on the device in FP64,
  * without downloading the state. */
 int nw_dg_rel_l2(nw_dg* s, int instance, const double* ref, size_t n, double* rel_l2);
+/* Extension (on-device ProbeManager, probe.cpp:33-216): places `n` probes at
+ * xyz[3q..3q+2] exactly as ProbeManager::init does (first element whose
+ * axis-aligned corner box is nearest, reference coordinates clamped to [-1,1],
+ * 1-D Lagrange weights on the GLL nodes), and from then on every step appends one
+ * record -- 6 field values per probe, summed in ProbeManager::record's order in
+ * FP64 -- to a device history of `capacity` records.  `m` is the mesh the solver
+ * was created from (single-mesh solvers only); n = 0 removes the probes. */
+int nw_dg_set_probes(nw_dg* s, const nw_mesh* m, int n, const double* xyz, int capacity);
+/* Appends one record of the current state (the reference records t = 0 before
+ * stepping, case.cpp:473-474). */
+int nw_dg_probe_sample(nw_dg* s);
+/* Copies the records taken since the last call, oldest first, as
+ * out[(r * n + q) * 6 + c] (c = Ex Ey Ez Hx Hy Hz), sets *nrecords, and starts the
+ * history again.  NW_ERR_STATE when more than `capacity` records were taken. */
+int nw_dg_probe_history(nw_dg* s, double* out, int max_records, int* nrecords);
+/* Host half of ProbeManager::init (probe.cpp:57-96): element and clamped (r,s,t)
+ * of each probe. */
+int nw_mesh_locate_probes(const nw_mesh* m, int n, const double* xyz, int* elem, double* rst);
 /* Autotuner: time both tile options per segment and keep the faster (runs by
  * itself before the first step when tile_option = -1). */
 int nw_dg_tune(nw_dg* s);

@@ -16,12 +16,17 @@
 #ifndef NEKWAVE_AMD_HPP
 #define NEKWAVE_AMD_HPP
 
+#include <sys/stat.h>
+
+#include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdint>
 #include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <array>
 #include <vector>
 
 #include "nekwave_amd.h"
@@ -160,11 +165,31 @@ class DgSolver {
         check(nw_dg_rel_l2(s_, 0, ref, n, &r));
         return r;
     }
+    // Extension: on-device ProbeManager (probe.cpp:33-216).  Every step after
+    // setProbes appends one record (6 values per probe) to a device history of
+    // `capacity` records; probeHistory copies and clears it.
+    void setProbes(const Mesh& mesh, const std::vector<std::array<double, 3>>& xyz, int capacity) {
+        check(nw_dg_set_probes(s_, mesh.handle(), static_cast<int>(xyz.size()),
+                               xyz.empty() ? nullptr : xyz[0].data(), capacity));
+        nprobes_ = static_cast<int>(xyz.size());
+        probe_cap_ = capacity;
+    }
+    void sampleProbes() { check(nw_dg_probe_sample(s_)); }
+    // records x probes x 6 values, oldest record first
+    std::vector<double> probeHistory(int* nrecords) {
+        std::vector<double> out(size_t(probe_cap_) * size_t(nprobes_ > 0 ? nprobes_ : 1) * 6);
+        int n = 0;
+        check(nw_dg_probe_history(s_, out.data(), probe_cap_, &n));
+        out.resize(size_t(n) * size_t(nprobes_) * 6);
+        if (nrecords) *nrecords = n;
+        return out;
+    }
     nw_opts& options() { return opts_; }
 
    private:
     nw_opts opts_;
     nw_dg* s_ = nullptr;
+    int nprobes_ = 0, probe_cap_ = 1;
 };
 
 using InitialConditionFn = std::function<void(double x, double y, double z, double& Ex, double& Ey, double& Ez,
@@ -187,6 +212,11 @@ struct Config {
     int pipeline = NW_PIPE_FUSED;
     double jitter = 0.0;
     uint64_t seed = 0;
+    // probes (config.hpp:46, 57, 291-322): absolute points, points relative to
+    // the box centre in units of its edge lengths, or num_probes generated ones
+    std::vector<std::array<double, 3>> probes, probeRel;
+    int numProbes = 0;
+    std::string outputDir = "output";  // "none" keeps probe records in memory only
 };
 
 class Case;
@@ -201,6 +231,32 @@ class Case {
     void setInitialCondition(InitialConditionFn fn) { ic_ = std::move(fn); }
     void setInitialConditionHook(InitialConditionFn fn) { setInitialCondition(std::move(fn)); }
     void setPostprocessingHook(PostprocessingFn fn) { post_ = std::move(fn); }
+    void addProbe(double x, double y, double z) { customProbes_.push_back({{x, y, z}}); }
+    void addRelativeProbe(double rx, double ry, double rz) { customRel_.push_back({{rx, ry, rz}}); }
+
+    // Probe coordinates in the order of case.cpp:383-422.
+    std::vector<std::array<double, 3>> probeList() const {
+        const Config& c = config_;
+        std::vector<std::array<double, 3>> p = c.probes;
+        p.insert(p.end(), customProbes_.begin(), customProbes_.end());
+        const double lx = c.xmax - c.xmin, ly = c.ymax - c.ymin, lz = c.zmax - c.zmin;
+        const double xc = 0.5 * (c.xmin + c.xmax), yc = 0.5 * (c.ymin + c.ymax), zc = 0.5 * (c.zmin + c.zmax);
+        std::vector<std::array<double, 3>> rel = c.probeRel;
+        rel.insert(rel.end(), customRel_.begin(), customRel_.end());
+        for (const auto& r : rel) p.push_back({{xc + r[0] * lx, yc + r[1] * ly, zc + r[2] * lz}});
+        if (p.empty() && c.numProbes > 0) {
+            if (c.nelx == 1 && c.nely == 1 && c.nelz == 1) {
+                const std::vector<double> z = mesh_->getGllZ();
+                for (int q = 0; q < std::min(c.numProbes, mesh_->getN()); ++q)
+                    p.push_back({{c.xmin + 0.5 * (1.0 + z[q]) * lx, yc, zc}});
+            } else {
+                const double x0 = c.xmin + 0.05 * lx, x1 = c.xmax - 0.05 * lx;
+                const double dx = c.numProbes > 1 ? (x1 - x0) / (c.numProbes - 1) : 0.0;
+                for (int q = 0; q < c.numProbes; ++q) p.push_back({{x0 + q * dx, yc, zc}});
+            }
+        }
+        return p;
+    }
 
     // case.cpp:98-524 (box meshes; dt rule :196-210; IC hook :227-246)
     void preprocess() {
@@ -248,6 +304,28 @@ class Case {
         const double finalTime = config_.finalTime;
         int step = 0;
         history_.clear();
+        probeSteps_.clear();
+        probeTimes_.clear();
+        probeValues_.clear();
+        probePoints_ = probeList();
+        std::vector<std::pair<int, double>> pending;
+        std::FILE* csv = nullptr;
+        if (!probePoints_.empty()) {
+            solver_->setProbes(*mesh_, probePoints_, outFreq + 1);
+            const std::string& d = config_.outputDir;
+            if (!d.empty() && d != "none" && d != "None" && d != "NONE") {  // probe.cpp:43-48, 117-133
+                ::mkdir(d.c_str(), 0755);
+                csv = std::fopen((d + "/probe_history.csv").c_str(), "w");
+                if (csv) {
+                    std::fputs("step,time", csv);
+                    for (size_t q = 1; q <= probePoints_.size(); ++q)
+                        std::fprintf(csv, ",Ex_%zu,Ey_%zu,Ez_%zu,Hx_%zu,Hy_%zu,Hz_%zu", q, q, q, q, q, q);
+                    std::fputs("\n", csv);
+                }
+            }
+            solver_->sampleProbes();  // t = 0 (case.cpp:473-474)
+            pending.push_back({0, 0.0});
+        }
         while (true) {
             if (maxSteps > 0 && step >= maxSteps) break;
             if (finalTime > 0.0 && currentTime_ >= finalTime - 1e-13) break;
@@ -257,10 +335,16 @@ class Case {
             solver_->step(dt, currentTime_);
             currentTime_ += dt;
             currentStep_ = step;
+            if (!probePoints_.empty()) pending.push_back({step, currentTime_});
             const bool last =
                 (maxSteps > 0 && step >= maxSteps) || (finalTime > 0.0 && currentTime_ >= finalTime - 1e-13);
-            if (step % outFreq == 0 || last) history_.push_back(solver_->diagnostics());
+            if (step % outFreq == 0 || last) {
+                history_.push_back(solver_->diagnostics());
+                fetchProbes(pending, csv);
+            }
         }
+        fetchProbes(pending, csv);
+        if (csv) std::fclose(csv);
         solver_->downloadState(state_.data(), state_.size());
     }
 
@@ -286,8 +370,32 @@ class Case {
     double currentTime() const { return currentTime_; }
     int currentStep() const { return currentStep_; }
     const std::vector<nw_diag>& history() const { return history_; }
+    // probe records: step, time and probes x 6 values (Ex Ey Ez Hx Hy Hz) each
+    const std::vector<std::array<double, 3>>& probePoints() const { return probePoints_; }
+    const std::vector<int>& probeSteps() const { return probeSteps_; }
+    const std::vector<double>& probeTimes() const { return probeTimes_; }
+    const std::vector<double>& probeValues() const { return probeValues_; }
 
    private:
+    void fetchProbes(std::vector<std::pair<int, double>>& pending, std::FILE* csv) {
+        if (pending.empty()) return;
+        int n = 0;
+        const std::vector<double> h = solver_->probeHistory(&n);
+        if (n != static_cast<int>(pending.size())) throw Error(NW_ERR_STATE, "probe history is short");
+        const size_t per = probePoints_.size() * 6;
+        for (int r = 0; r < n; ++r) {
+            probeSteps_.push_back(pending[r].first);
+            probeTimes_.push_back(pending[r].second);
+            probeValues_.insert(probeValues_.end(), h.begin() + long(r * per), h.begin() + long((r + 1) * per));
+            if (csv) {  // std::scientific, setprecision(8) (probe.cpp:133, 196-214)
+                std::fprintf(csv, "%d,%.8e", pending[r].first, pending[r].second);
+                for (size_t v = 0; v < per; ++v) std::fprintf(csv, ",%.8e", h[r * per + v]);
+                std::fputs("\n", csv);
+            }
+        }
+        pending.clear();
+    }
+
     Config config_;
     std::unique_ptr<Mesh> mesh_;
     std::unique_ptr<DgSolver> solver_;
@@ -295,6 +403,9 @@ class Case {
     InitialConditionFn ic_;
     PostprocessingFn post_;
     std::vector<nw_diag> history_;
+    std::vector<std::array<double, 3>> customProbes_, customRel_, probePoints_;
+    std::vector<int> probeSteps_;
+    std::vector<double> probeTimes_, probeValues_;
     double dt_ = 0.0, cfl_ = 0.0, currentTime_ = 0.0;
     int currentStep_ = 0;
 };

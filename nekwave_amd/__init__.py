@@ -203,6 +203,16 @@ class Mesh:
         check(lib().nw_mesh_corners(self.handle, _dp(c)))
         return c.reshape(-1, 8, 3)
 
+    def locateProbes(self, coords):
+        """ProbeManager::init's placement (probe.cpp:57-96): per probe the
+        element index and the clamped reference coordinates (r, s, t)."""
+        xyz = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1, 3)
+        n = xyz.shape[0]
+        elem = np.zeros(n, dtype=np.int32)
+        rst = np.zeros((n, 3))
+        check(lib().nw_mesh_locate_probes(self.handle, n, _dp(xyz), _ip(elem), _dp(rst)))
+        return elem, rst
+
     def computeMinNodeDistance(self):
         v = C.c_double()
         check(lib().nw_mesh_min_node_distance(self.handle, C.byref(v)))
@@ -289,6 +299,29 @@ class DgSolver:
         out = C.c_double()
         check(lib().nw_dg_rel_l2(self._handle(), int(instance), _dp(a), a.size, C.byref(out)))
         return out.value
+
+    def setProbes(self, mesh, coords, capacity=1024):
+        """On-device ProbeManager (probe.cpp:33-216): place probes at `coords`
+        ((n, 3) array) on `mesh`, the mesh this solver was created from; every
+        later step appends one record to a device history of `capacity` records."""
+        xyz = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1, 3)
+        check(lib().nw_dg_set_probes(self._handle(), mesh.handle, xyz.shape[0], _dp(xyz), int(capacity)))
+        self._nprobes = xyz.shape[0]
+        self._probe_cap = int(capacity)
+
+    def sampleProbes(self):
+        """Append one record of the current state (nw_dg_probe_sample)."""
+        check(lib().nw_dg_probe_sample(self._handle()))
+
+    def probeHistory(self):
+        """Records taken since the last call, oldest first: array (records,
+        probes, 6) with components Ex Ey Ez Hx Hy Hz."""
+        n = getattr(self, "_nprobes", 0)
+        cap = max(getattr(self, "_probe_cap", 1), 1)
+        out = np.zeros(cap * max(n, 1) * 6)
+        nrec = C.c_int()
+        check(lib().nw_dg_probe_history(self._handle(), _dp(out), cap, C.byref(nrec)))
+        return out[: nrec.value * n * 6].reshape(nrec.value, n, 6).copy()
 
     def info(self):
         o = _lib.DgInfo()

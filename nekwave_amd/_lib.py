@@ -78,6 +78,10 @@ SIGNATURES = {
     "nw_dg_sync": (i, [vp]),
     "nw_dg_diag": (i, [vp, C.POINTER(Diag)]),
     "nw_dg_rel_l2": (i, [vp, i, dp, C.c_size_t, dp]),
+    "nw_dg_set_probes": (i, [vp, vp, i, dp, i]),
+    "nw_dg_probe_sample": (i, [vp]),
+    "nw_dg_probe_history": (i, [vp, dp, i, ip]),
+    "nw_mesh_locate_probes": (i, [vp, i, dp, ip, dp]),
     "nw_dg_get_info": (i, [vp, C.POINTER(DgInfo)]),
     "nw_dg_destroy": (None, [vp]),
     "nw_batch_create": (i, [i, C.POINTER(vp), d, C.POINTER(Opts), C.POINTER(vp)]),

@@ -10,11 +10,17 @@ postprocess(): user hook, then DgSolver.finalize.
 
 Config keeps the subset of the reference's keys that shape the step
 (config.hpp:21-254): order, cfl, dt, c0/flux_type, nelx/nely/nelz, bounds,
-periodic_*, mesh file, num_steps/max_steps, final_time, output_frequency; plus the
-extensions precision, pipeline, jitter, seed.
+periodic_*, mesh file, num_steps/max_steps, final_time, output_frequency, probes,
+relative_probes, num_probes, output_dir; plus the extensions precision, pipeline,
+jitter, seed.
+
+Probes (probe.cpp:33-216, case.cpp:381-424, 473-474, 594-596) are recorded on the
+device every step; the small history comes back at output frequency and is
+written to <output_dir>/probe_history.csv in the reference's format.
 """
 import json
 import math
+import os
 import sys
 
 import numpy as np
@@ -33,7 +39,8 @@ class Config:
                 "max_steps": "num_steps", "t_final": "final_time", "final_time": "final_time",
                 "output_frequency": "output_frequency", "outputfreq": "output_frequency",
                 "flux": "flux_type", "mesh": "mesh_file", "file": "mesh_file", "path": "mesh_file",
-                "meshfile": "mesh_file"}
+                "meshfile": "mesh_file", "probe": "probes", "probe_rel": "relative_probes",
+                "outputdir": "output_dir"}
 
     def __init__(self, **kw):
         self.order = 3
@@ -51,6 +58,10 @@ class Config:
         self.pipeline = "fused"
         self.jitter = 0.0
         self.seed = 0
+        self.probes = []
+        self.relative_probes = []
+        self.num_probes = 0
+        self.output_dir = "output"
         for k, v in kw.items():
             self.set(k, v)
 
@@ -61,6 +72,15 @@ class Config:
             return
         if k in ("cfl", "dt") and isinstance(val, str) and val.lower() == "auto":
             setattr(self, k, -1.0)
+            return
+        if k in ("probes", "relative_probes"):  # config.hpp:291-322: [x, y, z] items or "x,y,z" strings
+            items = [val] if isinstance(val, str) else list(val)
+            if items and not isinstance(items[0], (list, tuple, str)):
+                items = [items]
+            for it in items:
+                v3 = [float(t) for t in it.replace(",", " ").split()] if isinstance(it, str) else [float(t) for t in it]
+                if len(v3) >= 3:
+                    getattr(self, k).append(tuple(v3[:3]))
             return
         if not hasattr(self, k):
             raise KeyError("unknown config key %r" % key)
@@ -107,6 +127,55 @@ class Case:
         self.current_time = 0.0
         self.current_step = 0
         self.history = []
+        self.custom_probes = []
+        self.custom_relative_probes = []
+        self.probe_points = np.zeros((0, 3))
+        self.probe_steps = []
+        self.probe_times = []
+        self.probe_values = np.zeros((0, 0, 6))
+
+    def addProbe(self, x, y, z):
+        self.custom_probes.append((float(x), float(y), float(z)))
+
+    def addRelativeProbe(self, rx, ry, rz):
+        self.custom_relative_probes.append((float(rx), float(ry), float(rz)))
+
+    def _probe_list(self):
+        """Probe coordinates in the order of case.cpp:383-422."""
+        c = self.config
+        pts = list(c.probes) + list(self.custom_probes)
+        lx, ly, lz = c.xmax - c.xmin, c.ymax - c.ymin, c.zmax - c.zmin
+        xc, yc, zc = 0.5 * (c.xmin + c.xmax), 0.5 * (c.ymin + c.ymax), 0.5 * (c.zmin + c.zmax)
+        for r in list(c.relative_probes) + list(self.custom_relative_probes):
+            pts.append((xc + r[0] * lx, yc + r[1] * ly, zc + r[2] * lz))
+        if not pts and c.num_probes > 0:
+            if c.nelx == 1 and c.nely == 1 and c.nelz == 1:
+                z = self.mesh.getGllZ()
+                for q in range(min(c.num_probes, self.mesh.getN())):
+                    pts.append((c.xmin + 0.5 * (1.0 + z[q]) * lx, yc, zc))
+            else:
+                x0, x1 = c.xmin + 0.05 * lx, c.xmax - 0.05 * lx
+                dx = (x1 - x0) / (c.num_probes - 1) if c.num_probes > 1 else 0.0
+                for q in range(c.num_probes):
+                    pts.append((x0 + q * dx, yc, zc))
+        return np.array(pts, dtype=np.float64).reshape(-1, 3)
+
+    def _fetch_probes(self, pending, out):
+        if not pending:
+            return
+        h = self.solver.probeHistory()
+        if h.shape[0] != len(pending):
+            raise RuntimeError("probe history holds %d records, expected %d" % (h.shape[0], len(pending)))
+        self._probe_chunks.append(h)
+        for (st, t), rec in zip(pending, h):
+            self.probe_steps.append(st)
+            self.probe_times.append(t)
+            if out is not None:
+                out.write("%d,%.8e" % (st, t))  # std::scientific, setprecision(8) (probe.cpp:133)
+                for v in rec.reshape(-1):
+                    out.write(",%.8e" % v)
+                out.write("\n")
+        pending.clear()
 
     def setInitialConditionHook(self, fn):
         """fn(x, y, z) -> (Ex, Ey, Ez, Hx, Hy, Hz), vectorised over node arrays."""
@@ -153,6 +222,20 @@ class Case:
         self.solver.uploadState(self.state)
         max_steps, final_time = c.num_steps, c.final_time
         out_freq = max(1, c.output_frequency)
+        self.probe_points = self._probe_list()
+        nprobe = self.probe_points.shape[0]
+        pending, out, self._probe_chunks = [], None, []
+        self.probe_steps, self.probe_times = [], []
+        if nprobe:
+            self.solver.setProbes(self.mesh, self.probe_points, capacity=out_freq + 1)
+            save = c.output_dir and c.output_dir.lower() != "none"
+            if save:  # probe.cpp:43-48, 117-133
+                os.makedirs(c.output_dir, exist_ok=True)
+                out = open(os.path.join(c.output_dir, "probe_history.csv"), "w")
+                out.write("step,time" + "".join(",Ex_%d,Ey_%d,Ez_%d,Hx_%d,Hy_%d,Hz_%d" % ((q + 1,) * 6)
+                                                 for q in range(nprobe)) + "\n")
+            self.solver.sampleProbes()  # t = 0 (case.cpp:473-474)
+            pending.append((0, 0.0))
         step = 0
         while True:  # case.cpp:562-583
             if max_steps > 0 and step >= max_steps:
@@ -166,10 +249,18 @@ class Case:
             self.solver.step(dt, self.current_time)
             self.current_time += dt
             self.current_step = step
+            if nprobe:
+                pending.append((step, self.current_time))
             final = (max_steps > 0 and step >= max_steps) or (final_time > 0.0 and self.current_time >= final_time - 1e-13)
             if step % out_freq == 0 or final:
                 d = self.solver.diagnostics()
                 self.history.append((step, self.current_time, d["max_e"], d["max_h"], d["energy"]))
+                self._fetch_probes(pending, out)
+        self._fetch_probes(pending, out)
+        if out is not None:
+            out.close()
+        self.probe_values = (np.concatenate(self._probe_chunks) if self._probe_chunks
+                             else np.zeros((0, nprobe, 6)))
         self.state = self.solver.downloadState()
 
     def postprocess(self):

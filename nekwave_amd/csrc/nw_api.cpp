@@ -265,6 +265,25 @@ NW_API int nw_dg_rel_l2(nw_dg* s, int instance, const double* ref, size_t n, dou
     NW_CHECK_ARG(s, "null solver");
     return s->s.rel_l2(instance, ref, n, rel_l2);
 }
+NW_API int nw_dg_set_probes(nw_dg* s, const nw_mesh* m, int n, const double* xyz, int capacity) {
+    NW_GUARD_BEGIN
+    NW_CHECK_ARG(s && m, "null argument");
+    return s->s.set_probes(m->m, n, xyz, capacity);
+    NW_GUARD_END
+}
+NW_API int nw_dg_probe_sample(nw_dg* s) {
+    NW_CHECK_ARG(s, "null solver");
+    return s->s.probe_sample();
+}
+NW_API int nw_dg_probe_history(nw_dg* s, double* out, int max_records, int* nrecords) {
+    NW_CHECK_ARG(s, "null solver");
+    return s->s.probe_history(out, max_records, nrecords);
+}
+NW_API int nw_mesh_locate_probes(const nw_mesh* m, int n, const double* xyz, int* elem, double* rst) {
+    NW_CHECK_ARG(m && n >= 0 && (n == 0 || (xyz && elem && rst)), "null argument");
+    nw::locate_probes(m->m, n, xyz, elem, rst);
+    return NW_OK;
+}
 NW_API int nw_dg_diag_instance(nw_dg* s, int instance, nw_diag* out) {
     NW_CHECK_ARG(s && out, "null argument");
     return s->s.diag(instance, out);

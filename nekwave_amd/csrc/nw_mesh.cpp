@@ -479,4 +479,49 @@ Mesh* Mesh::from_corners(int n1, int nelt, const double* corners, const int* kin
     return m;
 }
 
+void locate_probes(const Mesh& m, int n, const double* xyz, int* elem, double* rst) {
+    for (int q = 0; q < n; ++q) {
+        const double px = xyz[3 * q], py = xyz[3 * q + 1], pz = xyz[3 * q + 2];
+        int best = 0;
+        double bd = 1e20, br = 0.0, bs = 0.0, bt = 0.0;
+        for (int e = 0; e < m.nelt; ++e) {
+            const double* c = &m.corners[size_t(e) * 24];  // corner a, axis d at c[3a + d]
+            const double x0 = c[0], x1 = c[3], y0 = c[1], y1 = c[7], z0 = c[2], z1 = c[14];
+            const double r = std::fabs(x1 - x0) > 1e-14 ? 2.0 * (px - x0) / (x1 - x0) - 1.0 : 0.0;
+            const double s = std::fabs(y1 - y0) > 1e-14 ? 2.0 * (py - y0) / (y1 - y0) - 1.0 : 0.0;
+            const double t = std::fabs(z1 - z0) > 1e-14 ? 2.0 * (pz - z0) / (z1 - z0) - 1.0 : 0.0;
+            const double dr = std::max(0.0, std::max(-1.0 - r, r - 1.0));
+            const double ds = std::max(0.0, std::max(-1.0 - s, s - 1.0));
+            const double dt = std::max(0.0, std::max(-1.0 - t, t - 1.0));
+            const double d2 = dr * dr + ds * ds + dt * dt;
+            if (d2 < bd) {
+                bd = d2;
+                best = e;
+                br = std::max(-1.0, std::min(1.0, r));
+                bs = std::max(-1.0, std::min(1.0, s));
+                bt = std::max(-1.0, std::min(1.0, t));
+            }
+        }
+        elem[q] = best;
+        rst[3 * q] = br;
+        rst[3 * q + 1] = bs;
+        rst[3 * q + 2] = bt;
+    }
+}
+
+void lagrange_1d(int n1, const double* z, double xi, double* L) {
+    for (int i = 0; i < n1; ++i) {
+        if (std::fabs(xi - z[i]) < 1e-12) {
+            for (int j = 0; j < n1; ++j) L[j] = 0.0;
+            L[i] = 1.0;
+            return;
+        }
+    }
+    for (int i = 0; i < n1; ++i) {
+        L[i] = 1.0;
+        for (int j = 0; j < n1; ++j)
+            if (i != j) L[i] *= (xi - z[j]) / (z[i] - z[j]);
+    }
+}
+
 }  // namespace nw

@@ -44,6 +44,15 @@ struct Mesh {
     double min_node_distance() const;
 };
 
+// Probe placement of ProbeManager::init (probe.cpp:42-103): per target point,
+// the element whose axis-aligned corner box (corners 0, 1, 2, 4) is nearest --
+// the first one on ties -- and the point's reference coordinates in it,
+// clamped to [-1, 1].
+void locate_probes(const Mesh& m, int n, const double* xyz, int* elem, double* rst);
+// 1-D Lagrange basis through the GLL nodes at xi (probe.cpp:10-27): the unit
+// vector of a node within 1e-12 of xi, else the product formula.
+void lagrange_1d(int n1, const double* z, double xi, double* L);
+
 }  // namespace nw
 
 struct nw_mesh {

@@ -436,6 +436,40 @@ __global__ void k_l2_final(const double* __restrict__ part, int n, double* __res
     }
 }
 
+// Probe record (ProbeManager::record, probe.cpp:192-216) on the device: one
+// thread per (probe, component) sums w_p u_c over the probe element's nodes in
+// node order, w_p = (Lr[i] Ls[j]) Lt[k] -- the reference's weights and
+// summation order, in FP64 -- into record slot (*slot mod cap) of the history;
+// thread 0 then advances the slot.  One block, so no other block reads the
+// slot while it moves.
+template <typename R>
+__global__ void k_probe(const R* __restrict__ u, long long cs, long long ns, int nprobes,
+                        const long long* __restrict__ base, const double* __restrict__ lw, int n1,
+                        double* __restrict__ hist, int* __restrict__ slot, int cap) {
+    const int rec = *slot % cap;
+    const int np = n1 * n1 * n1;
+    for (int x = threadIdx.x; x < 6 * nprobes; x += blockDim.x) {
+        const int q = x / 6, c = x - 6 * q;
+        const double* Lr = lw + size_t(q) * 3 * n1;
+        const double* Ls = Lr + n1;
+        const double* Lt = Ls + n1;
+        const R* up = u + base[q] * ns + c * cs;
+        double acc = 0.0;
+        for (int p = 0; p < np; ++p) {
+            const int i = p % n1, j = (p / n1) % n1, kk = p / (n1 * n1);
+            const double w = __dmul_rn(__dmul_rn(Lr[i], Ls[j]), Lt[kk]);
+            acc = __dadd_rn(acc, __dmul_rn(w, double(up[p * ns])));  // no contraction: the host sum's roundings
+        }
+        hist[(size_t(rec) * nprobes + q) * 6 + c] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *slot = *slot + 1;
+}
+
+// Sets *slot = 0 with a kernel (no memset node if a caller captures it).
+__global__ void k_zero_int(int* p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
+}
 // ---------------------------------------------------------------------------
 // conversions and plain copy (HBM calibration)
 // ---------------------------------------------------------------------------
@@ -620,5 +654,23 @@ hipError_t launch_fill(double* p, long long n, double v, hipStream_t s) {
     hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kBlock), 0, s, p, n, v);
     return hipGetLastError();
 }
+
+template <typename R>
+hipError_t launch_probe(const R* u, long long cs, long long ns, int nprobes, const long long* base, const double* lw,
+                        int n1, double* hist, int* slot, int cap, hipStream_t s) {
+    if (nprobes <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_probe<R>, dim3(1), dim3(256), 0, s, u, cs, ns, nprobes, base, lw, n1, hist, slot, cap);
+    return hipGetLastError();
+}
+template hipError_t launch_probe<double>(const double*, long long, long long, int, const long long*, const double*, int,
+                                         double*, int*, int, hipStream_t);
+template hipError_t launch_probe<float>(const float*, long long, long long, int, const long long*, const double*, int,
+                                        double*, int*, int, hipStream_t);
+
+hipError_t launch_zero_int(int* p, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, p);
+    return hipGetLastError();
+}
+
 
 }  // namespace nw

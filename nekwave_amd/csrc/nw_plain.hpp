@@ -51,6 +51,12 @@ hipError_t launch_diag(const R* u, long long cs, long long ns, long long npts, i
 template <typename R>
 hipError_t launch_l2(const R* u, long long cs, long long ns, const double* ref, long long npts, int n1, const double* jac,
                      const ElemGeom* geom, double* part, double* out, hipStream_t s);
+// One probe record of the state: history[(*slot mod cap)][probe][component],
+// then ++*slot (one block; weights lw = Lr, Ls, Lt per probe, 3 n1 doubles).
+template <typename R>
+hipError_t launch_probe(const R* u, long long cs, long long ns, int nprobes, const long long* base, const double* lw,
+                        int n1, double* hist, int* slot, int cap, hipStream_t s);
+hipError_t launch_zero_int(int* p, hipStream_t s);
 template <typename D, typename S>
 hipError_t launch_convert(D* dst, long long dcs, long long dns, const S* src, long long scs, long long sns, long long n,
                           int ncomp, hipStream_t s);

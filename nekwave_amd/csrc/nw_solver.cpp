@@ -89,12 +89,7 @@ void Solver::release() {
     if (device < 0) return;
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
-    for (int p = 0; p < 2; ++p) {
-        if (gexec[p]) (void)hipGraphExecDestroy(gexec[p]);
-        if (graph[p]) (void)hipGraphDestroy(graph[p]);
-        gexec[p] = nullptr;
-        graph[p] = nullptr;
-    }
+    drop_graphs();
     for (void* p : allocs) (void)hipFree(p);
     allocs.clear();
     if (ev0) (void)hipEventDestroy(ev0);
@@ -117,6 +112,17 @@ int Solver::dalloc(void** p, size_t bytes) {
     allocs.push_back(*p);
     device_bytes += (long long)bytes;
     return NW_OK;
+}
+
+// Frees one buffer obtained from dalloc (no-op for null).
+void Solver::dfree(void* p) {
+    if (!p) return;
+    for (size_t i = 0; i < allocs.size(); ++i)
+        if (allocs[i] == p) {
+            allocs.erase(allocs.begin() + long(i));
+            (void)hipFree(p);
+            return;
+        }
 }
 
 template <typename T>
@@ -502,6 +508,113 @@ int Solver::enqueue_step(int parity) {
         else st = enqueue_plain_stage(s);
         if (st) return st;
     }
+    // the step's result sits in d_u[parity ^ 1] (fused) or d_state (plain)
+    return nprobes > 0 ? enqueue_probe(parity ^ 1) : NW_OK;
+}
+
+// One probe record of the state in d_u[state_parity] (fused) or d_state (plain).
+int Solver::enqueue_probe(int state_parity) {
+    if (pipe == NW_PIPE_PLAIN) {
+        NW_TRY_HIP(launch_probe<double>(d_state, stride, 1, nprobes, d_probe_base, d_probe_w, n1, d_probe_hist,
+                                        d_probe_slot, probe_cap, stream));
+    } else if (prec == kFP64) {
+        NW_TRY_HIP(launch_probe<double>(static_cast<const double*>(d_u[state_parity]), 1, 6, nprobes, d_probe_base,
+                                        d_probe_w, n1, d_probe_hist, d_probe_slot, probe_cap, stream));
+    } else {
+        NW_TRY_HIP(launch_probe<float>(static_cast<const float*>(d_u[state_parity]), 1, 6, nprobes, d_probe_base,
+                                       d_probe_w, n1, d_probe_hist, d_probe_slot, probe_cap, stream));
+    }
+    return NW_OK;
+}
+
+void Solver::drop_graphs() {
+    for (int p = 0; p < 2; ++p) {
+        if (gexec[p]) (void)hipGraphExecDestroy(gexec[p]);
+        if (graph[p]) (void)hipGraphDestroy(graph[p]);
+        gexec[p] = nullptr;
+        graph[p] = nullptr;
+    }
+}
+
+// ProbeManager::init (probe.cpp:42-103) for the device: place the probes on the
+// host exactly as the reference does, keep per probe its element's first node
+// and the 1-D Lagrange weights, and a history of `capacity` records that every
+// step appends to (the step graphs are rebuilt to include the record).
+int Solver::set_probes(const Mesh& m, int n, const double* xyz, int capacity) {
+    if (!initialized) { set_error("set_probes before create"); return NW_ERR_STATE; }
+    NW_CHECK_ARG(n >= 0 && (n == 0 || xyz != nullptr), "bad probe list");
+    NW_CHECK_ARG(capacity >= 1, "probe history capacity must be >= 1");
+    NW_CHECK_ARG(inst.size() == 1, "probes need a single-mesh solver");
+    NW_CHECK_ARG(m.n1 == n1 && m.nelt == nelt, "probes: mesh does not match the solver's mesh");
+    NW_CHECK_ARG(6.0 * double(n) <= 256.0 * 64.0, "too many probes (at most 2730)");
+    NW_TRY_HIP(hipSetDevice(device));
+    NW_TRY_HIP(hipStreamSynchronize(stream));
+    drop_graphs();
+    nprobes = 0;
+    dfree(d_probe_base);
+    dfree(d_probe_w);
+    dfree(d_probe_hist);
+    d_probe_base = nullptr;
+    d_probe_w = nullptr;
+    d_probe_hist = nullptr;
+    if (n == 0) return NW_OK;
+    std::vector<int> elem(n);
+    std::vector<double> rst(3 * size_t(n));
+    locate_probes(m, n, xyz, elem.data(), rst.data());
+    const GllTables& t = host_gll_tables();
+    std::vector<long long> base(n);
+    std::vector<double> lw(size_t(n) * 3 * n1);
+    for (int q = 0; q < n; ++q) {
+        base[q] = (long long)elem[q] * np;
+        for (int d = 0; d < 3; ++d) lagrange_1d(n1, t.z[n1], rst[3 * q + d], &lw[(size_t(q) * 3 + d) * n1]);
+    }
+    int st;
+    if ((st = upload_vec(&d_probe_base, base))) return st;
+    if ((st = upload_vec(&d_probe_w, lw))) return st;
+    if ((st = dalloc(reinterpret_cast<void**>(&d_probe_hist), sizeof(double) * 6 * size_t(n) * capacity))) return st;
+    if (!d_probe_slot && (st = dalloc(reinterpret_cast<void**>(&d_probe_slot), sizeof(int)))) return st;
+    NW_TRY_HIP(launch_zero_int(d_probe_slot, stream));
+    NW_TRY_HIP(hipStreamSynchronize(stream));
+    nprobes = n;
+    probe_cap = capacity;
+    return NW_OK;
+}
+
+// Records the current state once (the reference records t = 0 before stepping,
+// case.cpp:473-474).
+int Solver::probe_sample() {
+    if (!initialized || !uploaded) { set_error("probe_sample before create/upload"); return NW_ERR_STATE; }
+    if (nprobes <= 0) { set_error("no probes set"); return NW_ERR_STATE; }
+    NW_TRY_HIP(hipSetDevice(device));
+    int st = enqueue_probe(cur);
+    if (st) return st;
+    NW_TRY_HIP(hipStreamSynchronize(stream));
+    return NW_OK;
+}
+
+// Downloads the records taken since the last call (oldest first) and starts
+// the history again; more records than the capacity is an error (the oldest
+// were overwritten).
+int Solver::probe_history(double* host, int max_records, int* nrecords) {
+    if (!initialized) { set_error("probe_history before create"); return NW_ERR_STATE; }
+    NW_CHECK_ARG(nrecords != nullptr, "null record count");
+    if (nprobes <= 0) { *nrecords = 0; return NW_OK; }
+    NW_TRY_HIP(hipSetDevice(device));
+    int taken = 0;
+    NW_TRY_HIP(hipMemcpyAsync(&taken, d_probe_slot, sizeof(int), hipMemcpyDeviceToHost, stream));
+    NW_TRY_HIP(hipStreamSynchronize(stream));
+    if (taken > probe_cap) {
+        set_error("probe history overflow: fetch at least every capacity steps");
+        return NW_ERR_STATE;
+    }
+    NW_CHECK_ARG(host != nullptr || taken == 0, "null host buffer");
+    NW_CHECK_ARG(max_records >= taken, "host buffer holds fewer records than were taken");
+    if (taken > 0)
+        NW_TRY_HIP(hipMemcpyAsync(host, d_probe_hist, sizeof(double) * 6 * size_t(nprobes) * taken,
+                                  hipMemcpyDeviceToHost, stream));
+    NW_TRY_HIP(launch_zero_int(d_probe_slot, stream));
+    NW_TRY_HIP(hipStreamSynchronize(stream));
+    *nrecords = taken;
     return NW_OK;
 }
 

@@ -60,6 +60,13 @@ struct Solver {
     int *d_vM = nullptr, *d_vP = nullptr, *d_bc = nullptr;
     double* d_face[4] = {};  // nx ny nz dA
     double *d_fEN = nullptr, *d_fHN = nullptr, *d_flux = nullptr, *d_eps = nullptr, *d_mu = nullptr;
+    // probes (device-side ProbeManager): per probe the node base of its element and
+    // the 1-D Lagrange weights Lr, Ls, Lt; a history of `probe_cap` records
+    int nprobes = 0, probe_cap = 0;
+    long long* d_probe_base = nullptr;
+    double* d_probe_w = nullptr;
+    double* d_probe_hist = nullptr;
+    int* d_probe_slot = nullptr;
     // graphs per starting parity
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -74,10 +81,14 @@ struct Solver {
     int time(double dt, int nsteps, double* ms);
     int diag(int instance, nw_diag* out);
     int rel_l2(int instance, const double* ref, size_t n, double* out);
+    int set_probes(const Mesh& m, int n, const double* xyz, int capacity);
+    int probe_sample();
+    int probe_history(double* host, int max_records, int* nrecords);
     int tune();
 
    private:
     int dalloc(void** p, size_t bytes);
+    void dfree(void* p);
     template <typename T>
     int upload_vec(T** dst, const std::vector<T>& v);
     int init_plain(const Mesh& m);
@@ -87,6 +98,8 @@ struct Solver {
     int enqueue_plain_stage(int s);
     int enqueue_step(int parity);
     int build_graph(int parity);
+    int enqueue_probe(int state_parity);
+    void drop_graphs();
 };
 
 }  // namespace nw

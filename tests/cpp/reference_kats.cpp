@@ -87,7 +87,10 @@ static void BoxPecCase() {
     cfg.maxSteps = 50;
     cfg.outputFreq = 10;
     cfg.precision = g_prec;
+    cfg.outputDir = "none";
+    cfg.probeRel = {{{0.1, 0.2, -0.1}}};
     c.setConfig(cfg);
+    c.addProbe(0.3, -0.4, 0.25);
     c.setInitialConditionHook([](double x, double y, double z, double& ex, double& ey, double& ez, double& hx,
                                  double& hy, double& hz) {
         const double is6 = 1.0 / std::sqrt(6.0);
@@ -125,6 +128,11 @@ static void BoxPecCase() {
     EXPECT_TRUE(c.currentStep() == 50);
     EXPECT_TRUE(c.history().size() == 5);
     EXPECT_TRUE(rel <= 0.05);  // 3dboxpec_test.cpp:136
+    // probes: t = 0 and every step recorded on the device, fetched every 10 steps
+    EXPECT_TRUE(c.probePoints().size() == 2);
+    EXPECT_TRUE(c.probeSteps().size() == 51 && c.probeSteps().back() == 50);
+    EXPECT_TRUE(c.probeValues().size() == 51 * 2 * 6);
+    EXPECT_TRUE(std::isfinite(c.probeValues().back()) && c.probeValues()[5] != 0.0);
     std::printf("  PASSED\n");
 }
 

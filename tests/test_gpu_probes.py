@@ -1,0 +1,162 @@
+"""On-device probes (ProbeManager, probe.cpp:33-216) through the C ABI.
+
+Two checks pin the device records:
+  * exactness: on an affine box mesh a field polynomial of degree <= N-1 per
+    coordinate is reproduced exactly by GLL interpolation, so a record of such a
+    state equals the polynomial at the (clamped) probe point;
+  * parity with ProbeManager::record: every per-step record equals the
+    reference's sum, w = Lr[i] Ls[j] Lt[k], acc += w * u in node order in FP64,
+    evaluated on the host on the state downloaded after the same step --
+    bit-exact (same operations, same order, no contraction).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import nekwave_amd as nw
+from nekwave_amd.case import Case, Config
+from gpu_util import pair, random_state
+
+pytestmark = pytest.mark.gpu
+
+
+def _lagrange(z, xi):
+    """probe.cpp:12-30 (the delta shortcut within 1e-12 of a node)."""
+    n = len(z)
+    for i in range(n):
+        if abs(xi - z[i]) < 1e-12:
+            return [1.0 if j == i else 0.0 for j in range(n)]
+    out = []
+    for i in range(n):
+        v = 1.0
+        for j in range(n):
+            if i != j:
+                v = v * ((xi - z[j]) / (z[i] - z[j]))
+        out.append(v)
+    return out
+
+
+def _host_record(mesh, pts, state):
+    n1 = mesh.getN()
+    np_ = n1 ** 3
+    npts = mesh.getTotalPoints()
+    z = [float(v) for v in mesh.getGllZ()]
+    elem, rst = mesh.locateProbes(pts)
+    out = np.zeros((len(pts), 6))
+    for q in range(len(pts)):
+        L = [_lagrange(z, float(rst[q, d])) for d in range(3)]
+        off = int(elem[q]) * np_
+        for c in range(6):
+            u = state[c * npts + off: c * npts + off + np_].tolist()
+            acc = 0.0
+            for p in range(np_):
+                i, j, k = p % n1, (p // n1) % n1, p // (n1 * n1)
+                acc = acc + ((L[0][i] * L[1][j]) * L[2][k]) * u[p]
+            out[q, c] = acc
+    return out
+
+
+PTS = np.array([[0.13, 0.52, 0.77], [0.5, 0.5, 0.5], [0.999, 0.01, 0.3], [-0.2, 0.4, 1.3], [0.25, 0.25, 0.25]])
+
+
+@pytest.mark.parametrize("pipeline,precision", [("fused", "fp64"), ("fused", "fp32"), ("plain", "fp64")])
+def test_probe_polynomial_exact(pipeline, precision):
+    n1 = 5
+    m, o = pair(n1, (2, 3, 2))
+    x, y, zc = o.x, o.y, o.zc
+    deg = n1 - 1
+    f = [x ** deg * y + zc, y ** deg - x * zc, zc ** deg + x * y, x * y * zc, (x - y) ** 2, 1.0 + 0 * x]
+    st = np.concatenate(f)
+    s = nw.DgSolver(precision=precision, pipeline=pipeline)
+    s.initialize(m, 1.0)
+    s.uploadState(st)
+    s.setProbes(m, PTS, capacity=4)
+    s.sampleProbes()
+    h = s.probeHistory()
+    s.finalize()
+    assert h.shape == (1, len(PTS), 6)
+    cp = np.clip(PTS, 0.0, 1.0)
+    X, Y, Z = cp[:, 0], cp[:, 1], cp[:, 2]
+    want = np.stack([X ** deg * Y + Z, Y ** deg - X * Z, Z ** deg + X * Y, X * Y * Z, (X - Y) ** 2, 1.0 + 0 * X], 1)
+    tol = 1e-12 if precision == "fp64" else 2e-6
+    err = np.max(np.abs(h[0] - want))
+    print("%s %s max |probe - polynomial| = %.3e" % (pipeline, precision, err))
+    assert err <= tol, err
+
+
+@pytest.mark.parametrize("pipeline,precision,use_graph", [("fused", "fp64", True), ("fused", "fp32", True),
+                                                          ("fused", "fp64", False), ("plain", "fp64", True)])
+def test_probe_records_match_host_record_each_step(pipeline, precision, use_graph):
+    m, o = pair(6, (3, 2, 2), (0, 1.5, 0, 1, 0, 1), jitter=0.08, seed=4)
+    st0 = random_state(o, 11)
+    dt = 0.5 * o.auto_dt()
+    pts = np.array([[0.2, 0.3, 0.4], [1.4, 0.9, 0.1], [0.75, 0.5, 0.5]])
+    s = nw.DgSolver(precision=precision, pipeline=pipeline, use_graph=use_graph)
+    s.initialize(m, 1.0)
+    s.uploadState(st0)
+    s.setProbes(m, pts, capacity=8)
+    states = [s.downloadState()]
+    s.sampleProbes()
+    for _ in range(3):
+        s.step(dt)
+        states.append(s.downloadState())
+    s.run(dt, 2)  # several steps in one call record one per step too
+    states.append(None)
+    states.append(s.downloadState())
+    h = s.probeHistory()
+    assert h.shape == (6, len(pts), 6)
+    for r, stt in enumerate(states):
+        if stt is None:
+            continue
+        want = _host_record(m, pts, stt)
+        assert np.array_equal(h[r], want), (r, np.max(np.abs(h[r] - want)))
+    # the history starts again after a fetch
+    s.step(dt)
+    h2 = s.probeHistory()
+    assert h2.shape == (1, len(pts), 6)
+    np.testing.assert_array_equal(h2[0], _host_record(m, pts, s.downloadState()))
+    s.finalize()
+
+
+def test_probe_history_overflow_is_an_error():
+    m, o = pair(4, (2, 2, 2))
+    s = nw.DgSolver(precision="fp64")
+    s.initialize(m, 1.0)
+    s.uploadState(random_state(o, 2))
+    s.setProbes(m, PTS[:2], capacity=2)
+    s.run(0.5 * o.auto_dt(), 3)
+    with pytest.raises(nw.NekWaveError):
+        s.probeHistory()
+    s.setProbes(m, PTS[:2], capacity=2)  # placing them again starts a new history
+    s.step(0.5 * o.auto_dt())
+    assert s.probeHistory().shape == (1, 2, 6)
+    s.setProbes(m, np.zeros((0, 3)), capacity=1)  # removes the probes
+    s.step(0.5 * o.auto_dt())
+    assert s.probeHistory().shape == (0, 0, 6)
+    s.finalize()
+
+
+def test_case_writes_probe_history_csv(tmp_path):
+    """Case records t = 0 and every step (case.cpp:473-474, 594-596) and writes
+    probe_history.csv in ProbeManager's format (probe.cpp:117-133, 192-216)."""
+    out_dir = str(tmp_path / "out")
+    cfg = Config(order=5, nelx=2, nely=2, nelz=2, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax=1.0,
+                 num_steps=7, output_frequency=3, output_dir=out_dir, probes=[[0.3, 0.6, 0.2]],
+                 relative_probes=[[0.0, 0.0, 0.0]])
+    case = Case(cfg)
+    case.setInitialConditionHook(lambda x, y, z: (0 * x, 0 * x, np.sin(np.pi * x) * np.sin(np.pi * y), 0 * x,
+                                                  0 * x, 0 * x))
+    case.preprocess()
+    case.simulate()
+    assert case.probe_steps == list(range(8))
+    assert case.probe_values.shape == (8, 2, 6)
+    np.testing.assert_array_equal(case.probe_values[-1], _host_record(case.mesh, case.probe_points, case.state))
+    lines = open(os.path.join(out_dir, "probe_history.csv")).read().splitlines()
+    assert lines[0] == "step,time," + ",".join("%s_%d" % (c, q) for q in (1, 2)
+                                                for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"))
+    assert len(lines) == 9
+    row = lines[-1].split(",")
+    assert row[0] == "7" and row[1] == "%.8e" % case.probe_times[-1]
+    np.testing.assert_allclose([float(v) for v in row[2:]], case.probe_values[-1].reshape(-1), rtol=1e-8, atol=0)
+    case.postprocess()
