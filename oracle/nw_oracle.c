@@ -791,6 +791,83 @@ double or_solver_run(or_solver *s, double dt, int nsteps) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* probes                                                                    */
+/* ------------------------------------------------------------------------ */
+/* evalLagrange1D (probe.cpp:11-30): the delta vector within 1e-12 of a node,
+ * else the product over j != i of (xi - z_j) / (z_i - z_j), left to right. */
+void or_lagrange_1d(int n1, const double *z, double xi, double *L) {
+    for (int i = 0; i < n1; ++i)
+        if (fabs(xi - z[i]) < 1e-12) {
+            for (int j = 0; j < n1; ++j) L[j] = 0.0;
+            L[i] = 1.0;
+            return;
+        }
+    for (int i = 0; i < n1; ++i) {
+        L[i] = 1.0;
+        for (int j = 0; j < n1; ++j)
+            if (j != i) L[i] *= (xi - z[j]) / (z[i] - z[j]);
+    }
+}
+
+/* ProbeManager::init's search (probe.cpp:64-91): per element the reference
+ * coordinates from the axis-aligned box of corners 0, 1 (x), 2 (y), 4 (z);
+ * the squared distance of (r,s,t) to [-1,1]^3; the first element with the
+ * smallest distance wins (strict <), its (r,s,t) clamped to [-1,1]. */
+void or_locate_probes(const or_mesh *m, int n, const double *xyz, int *elem, double *rst) {
+    for (int q = 0; q < n; ++q) {
+        int best = 0;
+        double bestd = 1e20, br = 0.0, bs = 0.0, bt = 0.0;
+        for (int e = 0; e < m->nelt; ++e) {
+            const double *c = m->corners + (size_t)e * 24;
+            double x0 = c[0], x1 = c[1 * 3 + 0], y0 = c[1], y1 = c[2 * 3 + 1], z0 = c[2], z1 = c[4 * 3 + 2];
+            double r = fabs(x1 - x0) > 1e-14 ? 2.0 * (xyz[3 * q + 0] - x0) / (x1 - x0) - 1.0 : 0.0;
+            double s = fabs(y1 - y0) > 1e-14 ? 2.0 * (xyz[3 * q + 1] - y0) / (y1 - y0) - 1.0 : 0.0;
+            double t = fabs(z1 - z0) > 1e-14 ? 2.0 * (xyz[3 * q + 2] - z0) / (z1 - z0) - 1.0 : 0.0;
+            double dr = fmax(0.0, fmax(-1.0 - r, r - 1.0));
+            double ds = fmax(0.0, fmax(-1.0 - s, s - 1.0));
+            double dt = fmax(0.0, fmax(-1.0 - t, t - 1.0));
+            double d2 = dr * dr + ds * ds + dt * dt;
+            if (d2 < bestd) {
+                bestd = d2;
+                best = e;
+                br = fmax(-1.0, fmin(1.0, r));
+                bs = fmax(-1.0, fmin(1.0, s));
+                bt = fmax(-1.0, fmin(1.0, t));
+            }
+        }
+        elem[q] = best;
+        rst[3 * q + 0] = br;
+        rst[3 * q + 1] = bs;
+        rst[3 * q + 2] = bt;
+    }
+}
+
+/* ProbeManager::record (probe.cpp:192-216) with the weights of probe.cpp:98-105:
+ * w = (Lr_i Ls_j) Lt_k over the element's nodes in storage order, each
+ * component summed as acc += w * u.  out[6q + c]. */
+void or_probe_record(const or_mesh *m, int n, const int *elem, const double *rst, const double *state,
+                     int stride, double *out) {
+    int N = m->n1;
+    size_t S = (size_t)stride;
+    for (int q = 0; q < n; ++q) {
+        double Lr[OR_MAXN1], Ls[OR_MAXN1], Lt[OR_MAXN1];
+        or_lagrange_1d(N, m->z, rst[3 * q + 0], Lr);
+        or_lagrange_1d(N, m->z, rst[3 * q + 1], Ls);
+        or_lagrange_1d(N, m->z, rst[3 * q + 2], Lt);
+        size_t off = (size_t)elem[q] * m->np;
+        double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int k = 0; k < N; ++k)
+            for (int j = 0; j < N; ++j)
+                for (int i = 0; i < N; ++i) {
+                    double w = Lr[i] * Ls[j] * Lt[k];
+                    size_t p = off + (size_t)(i + N * (j + N * k));
+                    for (int c = 0; c < 6; ++c) acc[c] += w * state[c * S + p];
+                }
+        for (int c = 0; c < 6; ++c) out[6 * q + c] = acc[c];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* diagnostics (case.cpp:733-794); eps/mu weighting when media are set       */
 /* ------------------------------------------------------------------------ */
 double or_energy(const or_mesh *m, const double *u, int stride) {

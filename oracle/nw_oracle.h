@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
  * algorithm (NekWave src/mesh.cpp, src/dg_solver.cpp, src/device/dg_kernels.cu,
- * src/case.cpp) used as the checker for the HIP product and as the timed CPU
+ * src/case.cpp, src/probe.cpp) used as the checker for the HIP product and as the timed CPU
  * baseline.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
  * leg may load it; the product library never links or calls it.
  *
@@ -106,6 +106,13 @@ double or_solver_run(or_solver *s, double dt, int nsteps); /* returns wall secon
 double or_energy(const or_mesh *m, const double *state, int stride);
 double or_max_e(const or_mesh *m, const double *state, int stride);
 double or_max_h(const or_mesh *m, const double *state, int stride);
+
+/* probes (probe.cpp:11-30, 57-105, 192-216): element and clamped (r,s,t) of
+ * each point, and one record (6 values per probe) of a state */
+void or_lagrange_1d(int n1, const double *z, double xi, double *L);
+void or_locate_probes(const or_mesh *m, int n, const double *xyz, int *elem, double *rst);
+void or_probe_record(const or_mesh *m, int n, const int *elem, const double *rst, const double *state,
+                     int stride, double *out);
 
 void or_set_threads(int n);
 int  or_get_threads(void);

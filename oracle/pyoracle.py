@@ -86,6 +86,9 @@ def lib():
             "or_energy": (d, [P(_Mesh), dp, i]),
             "or_max_e": (d, [P(_Mesh), dp, i]),
             "or_max_h": (d, [P(_Mesh), dp, i]),
+            "or_lagrange_1d": (None, [i, dp, d, dp]),
+            "or_locate_probes": (None, [P(_Mesh), i, dp, ip, dp]),
+            "or_probe_record": (None, [P(_Mesh), i, ip, dp, dp, i, dp]),
             "or_set_threads": (None, [i]),
             "or_rng_seed": (None, [C.c_void_p, u64]),
             "or_rng_next": (u64, [C.c_void_p]),
@@ -227,6 +230,23 @@ class Mesh:
     def energy(self, state, stride=None):
         state = np.ascontiguousarray(state, dtype=np.float64)
         return lib().or_energy(self._p, dptr(state), stride or self.npts)
+
+    def locate_probes(self, xyz):
+        """ProbeManager::init's placement (probe.cpp:64-91): element, (r, s, t)."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        n = xyz.shape[0]
+        elem = np.zeros(n, dtype=np.int32)
+        rst = np.zeros((n, 3))
+        lib().or_locate_probes(self._p, n, dptr(xyz), iptr(elem), dptr(rst))
+        return elem, rst
+
+    def probe_record(self, xyz, state, stride=None):
+        """ProbeManager::record (probe.cpp:192-216) of `state`: (probes, 6)."""
+        elem, rst = self.locate_probes(xyz)
+        state = np.ascontiguousarray(state, dtype=np.float64)
+        out = np.zeros((elem.size, 6))
+        lib().or_probe_record(self._p, elem.size, iptr(elem), dptr(rst), dptr(state), stride or self.npts, dptr(out))
+        return out
 
     def max_e(self, state):
         state = np.ascontiguousarray(state, dtype=np.float64)

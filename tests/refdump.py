@@ -10,10 +10,41 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DUMPER = os.path.join(ROOT, "oracle", "_ref", "ref_mesh_dump")
+PROBE_DUMPER = os.path.join(ROOT, "oracle", "_ref", "ref_probe_dump")
 
 
 def available():
     return os.path.exists(DUMPER)
+
+
+def probes_available():
+    return os.path.exists(PROBE_DUMPER)
+
+
+def probes(tmpdir, mesh_args, pts, name="probes"):
+    """ProbeManager::init of the reference on a box (mesh_args = ("box", n1, nel,
+    bounds)) or a .rea file (("rea", n1, path)): per probe the element offset
+    and the n1^3 interpolation weights."""
+    pp = os.path.join(str(tmpdir), name + "_pts.bin")
+    op = os.path.join(str(tmpdir), name + "_out.bin")
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    with open(pp, "wb") as fh:
+        fh.write(np.array([pts.shape[0]], np.int32).tobytes() + pts.tobytes())
+    if mesh_args[0] == "box":
+        _, n1, nel, bounds = mesh_args
+        args = [PROBE_DUMPER, "box", str(n1)] + [str(v) for v in nel] + [repr(float(b)) for b in bounds]
+    else:
+        _, n1, path = mesh_args
+        args = [PROBE_DUMPER, "rea", str(n1), path]
+    subprocess.run(args + [pp, op], check=True, stdout=subprocess.DEVNULL)
+    raw = open(op, "rb").read()
+    n, np_ = np.frombuffer(raw, np.int32, 2)
+    offs, wts, off = [], [], 8
+    for _ in range(int(n)):
+        offs.append(int(np.frombuffer(raw, np.int32, 1, off)[0]))
+        wts.append(np.frombuffer(raw, np.float64, int(np_), off + 4).copy())
+        off += 4 + 8 * int(np_)
+    return np.array(offs), np.array(wts)
 
 
 def _read(path):

@@ -4,10 +4,9 @@ Two checks pin the device records:
   * exactness: on an affine box mesh a field polynomial of degree <= N-1 per
     coordinate is reproduced exactly by GLL interpolation, so a record of such a
     state equals the polynomial at the (clamped) probe point;
-  * parity with ProbeManager::record: every per-step record equals the
-    reference's sum, w = Lr[i] Ls[j] Lt[k], acc += w * u in node order in FP64,
-    evaluated on the host on the state downloaded after the same step --
-    bit-exact (same operations, same order, no contraction).
+  * parity with ProbeManager::record: every per-step record equals the oracle's
+    restatement (w = Lr[i] Ls[j] Lt[k], acc += w * u in node order, FP64, no
+    contraction) of the state downloaded after the same step -- bit-exact.
 """
 import os
 
@@ -16,45 +15,15 @@ import pytest
 
 import nekwave_amd as nw
 from nekwave_amd.case import Case, Config
+import pyoracle as po
 from gpu_util import pair, random_state
 
 pytestmark = pytest.mark.gpu
 
 
-def _lagrange(z, xi):
-    """probe.cpp:12-30 (the delta shortcut within 1e-12 of a node)."""
-    n = len(z)
-    for i in range(n):
-        if abs(xi - z[i]) < 1e-12:
-            return [1.0 if j == i else 0.0 for j in range(n)]
-    out = []
-    for i in range(n):
-        v = 1.0
-        for j in range(n):
-            if i != j:
-                v = v * ((xi - z[j]) / (z[i] - z[j]))
-        out.append(v)
-    return out
-
-
-def _host_record(mesh, pts, state):
-    n1 = mesh.getN()
-    np_ = n1 ** 3
-    npts = mesh.getTotalPoints()
-    z = [float(v) for v in mesh.getGllZ()]
-    elem, rst = mesh.locateProbes(pts)
-    out = np.zeros((len(pts), 6))
-    for q in range(len(pts)):
-        L = [_lagrange(z, float(rst[q, d])) for d in range(3)]
-        off = int(elem[q]) * np_
-        for c in range(6):
-            u = state[c * npts + off: c * npts + off + np_].tolist()
-            acc = 0.0
-            for p in range(np_):
-                i, j, k = p % n1, (p // n1) % n1, p // (n1 * n1)
-                acc = acc + ((L[0][i] * L[1][j]) * L[2][k]) * u[p]
-            out[q, c] = acc
-    return out
+def _host_record(o, pts, state):
+    """The oracle's ProbeManager::record (probe.cpp:192-216) of a downloaded state."""
+    return o.probe_record(pts, state)
 
 
 PTS = np.array([[0.13, 0.52, 0.77], [0.5, 0.5, 0.5], [0.999, 0.01, 0.3], [-0.2, 0.4, 1.3], [0.25, 0.25, 0.25]])
@@ -109,13 +78,13 @@ def test_probe_records_match_host_record_each_step(pipeline, precision, use_grap
     for r, stt in enumerate(states):
         if stt is None:
             continue
-        want = _host_record(m, pts, stt)
+        want = _host_record(o, pts, stt)
         assert np.array_equal(h[r], want), (r, np.max(np.abs(h[r] - want)))
     # the history starts again after a fetch
     s.step(dt)
     h2 = s.probeHistory()
     assert h2.shape == (1, len(pts), 6)
-    np.testing.assert_array_equal(h2[0], _host_record(m, pts, s.downloadState()))
+    np.testing.assert_array_equal(h2[0], _host_record(o, pts, s.downloadState()))
     s.finalize()
 
 
@@ -151,7 +120,8 @@ def test_case_writes_probe_history_csv(tmp_path):
     case.simulate()
     assert case.probe_steps == list(range(8))
     assert case.probe_values.shape == (8, 2, 6)
-    np.testing.assert_array_equal(case.probe_values[-1], _host_record(case.mesh, case.probe_points, case.state))
+    o = po.Mesh.box(5, (2, 2, 2), (0.0, 1.0, 0.0, 1.0, 0.0, 1.0))
+    np.testing.assert_array_equal(case.probe_values[-1], _host_record(o, case.probe_points, case.state))
     lines = open(os.path.join(out_dir, "probe_history.csv")).read().splitlines()
     assert lines[0] == "step,time," + ",".join("%s_%d" % (c, q) for q in (1, 2)
                                                 for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"))

@@ -4,19 +4,38 @@ probe.cpp:33-116; Case::initProbes, case.cpp:381-424; config.hpp:291-322, 473-49
 The placement is pinned by what the geometry fixes: on an affine box mesh a
 point at known element coordinates must come back in that element with those
 (r, s, t); points on shared faces go to the lowest-index element (strict '<' in
-probe.cpp:84); points outside the box are clamped to its surface.
+probe.cpp:84); points outside the box are clamped to its surface.  The oracle's
+probe functions (or_lagrange_1d, or_locate_probes, or_probe_record) are pinned
+the same way and by interpolation identities (partition of unity, exact
+reproduction of polynomials of degree < n1), and the product's placement must
+equal the oracle's bit for bit on jittered meshes.
 """
 import numpy as np
 import pytest
 
+import ctypes as C
+
 import nekwave_amd as nw
+import pyoracle as po
+import refdump
 from nekwave_amd.case import Case, Config
+
+needs_probe_ref = pytest.mark.skipif(not refdump.probes_available(),
+                                     reason="oracle/_ref not built (no reference tree)")
 
 
 def _box(n1=5, nel=(3, 2, 4), bounds=(0.0, 1.5, -1.0, 1.0, 0.0, 2.0)):
     m = nw.Mesh(n1)
     m.createBoxMesh(*nel, *bounds)
     return m
+
+
+class _OracleLocate:
+    def __init__(self, o):
+        self.o = o
+
+    def locateProbes(self, pts):
+        return self.o.locate_probes(pts)
 
 
 def _elem_index(ex, ey, ez, nel):
@@ -48,9 +67,10 @@ def test_locate_interior_points_known_element_coordinates():
         np.testing.assert_allclose(rst[q], r, rtol=0, atol=1e-13)
 
 
-def test_locate_shared_face_goes_to_first_element_and_outside_is_clamped():
+@pytest.mark.parametrize("impl", ["product", "oracle"])
+def test_locate_shared_face_goes_to_first_element_and_outside_is_clamped(impl):
     nel, b = (2, 2, 2), (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
-    m = _box(4, nel, b)
+    m = _box(4, nel, b) if impl == "product" else _OracleLocate(po.Mesh.box(4, nel, b))
     pts = np.array([
         [0.5, 0.25, 0.25],   # on the x-face between elements 0 and 1
         [0.5, 0.5, 0.5],     # the centre vertex shared by all eight
@@ -107,3 +127,98 @@ def test_config_num_probes(nel):
     else:
         np.testing.assert_allclose(p[:, 0], np.linspace(0.1, 1.9, 7), atol=1e-15)
     np.testing.assert_allclose(p[:, 1:], 0.5, atol=0)
+
+
+def test_oracle_locate_matches_product_on_jittered_mesh():
+    nel, b = (4, 3, 3), (0.0, 2.0, -1.0, 1.0, 0.0, 1.5)
+    m = nw.Mesh(6)
+    m.createBoxMesh(*nel, *b, jitter=0.12, seed=7)
+    o = po.Mesh.box(6, nel, b, jitter=0.12, seed=7)
+    rng = np.random.default_rng(5)
+    pts = np.stack([rng.uniform(-0.3, 2.3, 200), rng.uniform(-1.2, 1.2, 200), rng.uniform(-0.2, 1.7, 200)], 1)
+    e1, r1 = m.locateProbes(pts)
+    e2, r2 = o.locate_probes(pts)
+    np.testing.assert_array_equal(e1, e2)
+    np.testing.assert_array_equal(r1, r2)
+
+
+@pytest.mark.parametrize("n1", [2, 5, 9])
+def test_oracle_lagrange_identities(n1):
+    z = po.gll(n1)[0]
+    L = np.zeros(n1)
+    for xi in list(np.linspace(-1, 1, 13)) + list(z):
+        po.lib().or_lagrange_1d(n1, po.dptr(np.ascontiguousarray(z)), C.c_double(xi), po.dptr(L))
+        assert abs(L.sum() - 1.0) < 1e-13
+        for deg in range(n1):
+            assert abs(np.dot(L, z ** deg) - xi ** deg) < 1e-13
+    po.lib().or_lagrange_1d(n1, po.dptr(np.ascontiguousarray(z)), C.c_double(z[1] + 5e-13), po.dptr(L))
+    assert L[1] == 1.0 and np.count_nonzero(L) == 1  # the node shortcut (probe.cpp:15-20)
+
+
+def test_oracle_probe_record_reproduces_polynomials():
+    n1 = 5
+    o = po.Mesh.box(n1, (3, 2, 2), (0.0, 1.5, 0.0, 1.0, -1.0, 1.0))
+    x, y, z = o.x, o.y, o.zc
+    deg = n1 - 1
+    fs = [lambda x, y, z: x ** deg * y + z, lambda x, y, z: y ** deg - x * z, lambda x, y, z: z ** deg + x * y,
+          lambda x, y, z: x ** 2 * y ** 3 * z, lambda x, y, z: (x - 2 * y) ** 3, lambda x, y, z: 1.0 + 0 * x]
+    st = np.concatenate([f(x, y, z) for f in fs])
+    pts = np.array([[0.1, 0.2, 0.3], [1.2, 0.7, -0.9], [0.75, 0.5, 0.0], [2.0, -1.0, 0.4]])
+    got = o.probe_record(pts, st)
+    cp = np.clip(pts, [0.0, 0.0, -1.0], [1.5, 1.0, 1.0])
+    want = np.stack([f(cp[:, 0], cp[:, 1], cp[:, 2]) for f in fs], 1)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
+
+
+def _oracle_weights(o, elem, rst):
+    """Per probe the oracle's element offset and weights (Lr_i Ls_j) Lt_k in storage order."""
+    n1 = o.n1
+    z = np.ascontiguousarray(o.z[:n1])
+    offs, wts = [], []
+    for q in range(elem.size):
+        L = []
+        for d in range(3):
+            v = np.zeros(n1)
+            po.lib().or_lagrange_1d(n1, po.dptr(z), C.c_double(rst[q, d]), po.dptr(v))
+            L.append(v)
+        w = (L[0][None, None, :] * L[1][None, :, None]) * L[2][:, None, None]
+        offs.append(int(elem[q]) * n1 ** 3)
+        wts.append(w.reshape(-1))
+    return np.array(offs), np.array(wts)
+
+
+_REF_PTS = np.array([[0.13, 0.52, 0.77], [0.5, 0.5, 0.5], [0.999, 0.01, 0.3], [-0.2, 0.4, 1.3],
+                     [0.25, 0.25, 0.25], [1.0 / 3.0, 2.0 / 3.0, 0.5], [0.0, 0.0, 0.0], [0.61, 0.07, 0.93]])
+
+
+@needs_probe_ref
+@pytest.mark.parametrize("n1,nel", [(4, (3, 3, 3)), (6, (2, 3, 2)), (9, (1, 1, 1))])
+def test_oracle_probe_weights_match_reference_box(tmp_path, n1, nel):
+    """The reference's ProbeManager::init (probe.cpp compiled in place) and the
+    oracle agree bit for bit on element offsets and interpolation weights."""
+    bounds = (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
+    ro, rw = refdump.probes(tmp_path, ("box", n1, nel, bounds), _REF_PTS)
+    o = po.Mesh.box(n1, nel, bounds)
+    oo, ow = _oracle_weights(o, *o.locate_probes(_REF_PTS))
+    np.testing.assert_array_equal(ro, oo)
+    np.testing.assert_array_equal(rw, ow)
+    m = nw.Mesh(n1)
+    m.createBoxMesh(*nel, *bounds)
+    e, _ = m.locateProbes(_REF_PTS)
+    np.testing.assert_array_equal(e * n1 ** 3, ro)
+
+
+@needs_probe_ref
+def test_oracle_probe_weights_match_reference_jittered_rea(tmp_path):
+    """Jittered corners through the reference's .rea loader: the first element
+    whose corner box is nearest wins, as in probe.cpp:64-91."""
+    n1 = 5
+    o = po.Mesh.box(n1, (3, 3, 3), (0, 1, 0, 1, 0, 1), jitter=0.12, seed=3)
+    rp = str(tmp_path / "j.rea")
+    refdump.write_rea(rp, o.corners, o.f_kind, o.f_nbr_e, o.f_nbr_f)
+    rng = np.random.default_rng(9)
+    pts = np.concatenate([_REF_PTS, rng.uniform(-0.1, 1.1, (40, 3))])
+    ro, rw = refdump.probes(tmp_path, ("rea", n1, rp), pts)
+    oo, ow = _oracle_weights(o, *o.locate_probes(pts))
+    np.testing.assert_array_equal(ro, oo)
+    np.testing.assert_array_equal(rw, ow)
