@@ -163,6 +163,49 @@ def tm110_c1(nw, W):
     return out
 
 
+def case_loop(nw, wl, precision, steps=100, out_freq=10):
+    """Case::simulate's loop on the same mesh, wall clock around the host loop:
+    the reference's pattern (the whole state downloaded after every step to
+    record probes and energy, case.cpp:586-598) against the on-device pattern
+    (probes recorded inside each step's graph, the steps between outputs
+    launched together, diagnostics and the probe history fetched every
+    `out_freq` steps, the state once at the end)."""
+    m = wl.meshes[0]
+    dof = 6 * m.getTotalPoints()
+    pts = np.array([[0.3, 0.5, 0.5], [0.5, 0.5, 0.5], [0.7, 0.5, 0.5], [0.5, 0.25, 0.75]])
+    host = np.zeros(dof)
+    out = {"steps": steps, "output_frequency": out_freq, "probes": len(pts)}
+    s = make_solver(nw, wl, precision)
+    s.run(wl.dt, 3)
+    nw.lib().nw_dev_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step(wl.dt)
+        s.downloadState(host)
+    t1 = time.perf_counter()
+    s.finalize()
+    out["download_every_step"] = {"dof_updates_per_s": dof * steps / (t1 - t0), "ms_per_step": 1e3 * (t1 - t0) / steps}
+    s = make_solver(nw, wl, precision)
+    s.setProbes(m, pts, capacity=out_freq + 1)
+    s.run(wl.dt, 3)
+    s.probeHistory()
+    nw.lib().nw_dev_sync()
+    t0 = time.perf_counter()
+    s.sampleProbes()
+    nrec = 0
+    for step in range(out_freq, steps + out_freq, out_freq):  # as Case: one launch per output interval
+        s.run(wl.dt, min(step, steps) - (step - out_freq))
+        s.diagnostics()
+        nrec += s.probeHistory().shape[0]
+    s.downloadState(host)
+    t1 = time.perf_counter()
+    s.finalize()
+    assert nrec == steps + 1
+    out["on_device"] = {"dof_updates_per_s": dof * steps / (t1 - t0), "ms_per_step": 1e3 * (t1 - t0) / steps}
+    out["speedup"] = out["on_device"]["dof_updates_per_s"] / out["download_every_step"]["dof_updates_per_s"]
+    return out
+
+
 def other_config(nw, W, name, steps):
     wl = W.BUILDERS[name]()
     out = {"nodes": wl.npts, "elements": wl.nelt, "instances": len(wl.meshes), "description": wl.info["description"]}
@@ -255,6 +298,7 @@ def main():
         acc = rel_l2(outs[args.precision], outs["fp64"], w)
 
     c1 = tm110_c1(nw, W)
+    loop = case_loop(nw, wl, args.precision)
 
     others = {}
     if args.configs and args.configs != "none":
@@ -298,6 +342,7 @@ def main():
         "ms_per_step_repeats": reps,
         "ms_per_step_median": float(np.median(reps)),
         "tm110_c1": c1,
+        "case_loop": loop,
         "other_configs": others,
         "wall_s_timed": wall,
         "cpu_baseline": cpu,

@@ -290,8 +290,9 @@ class Case {
         currentStep_ = 0;
     }
 
-    // case.cpp:529-660 without the per-step download: diagnostics are reduced
-    // on the device at output frequency and the state comes back once.
+    // case.cpp:529-660 without the per-step download: probes are recorded on the
+    // device every step, diagnostics are reduced on the device at output
+    // frequency, and the state comes back once.
     void simulate() {
         nw_opts o;
         nw_opts_default(&o);
@@ -326,23 +327,35 @@ class Case {
             solver_->sampleProbes();  // t = 0 (case.cpp:473-474)
             pending.push_back({0, 0.0});
         }
+        // The reference's step schedule (case.cpp:562-583); the steps between two
+        // outputs with equal dt go to the device as one multi-step launch.
+        double chunkDt = 0.0;
+        int chunkN = 0;
+        auto flush = [&]() {
+            if (chunkN) solver_->run(chunkDt, chunkN);
+            chunkN = 0;
+        };
         while (true) {
             if (maxSteps > 0 && step >= maxSteps) break;
             if (finalTime > 0.0 && currentTime_ >= finalTime - 1e-13) break;
             double dt = dt_;
             if (finalTime > 0.0 && currentTime_ + dt > finalTime) dt = finalTime - currentTime_;
+            if (chunkN && dt != chunkDt) flush();
+            chunkDt = dt;
+            ++chunkN;
             ++step;
-            solver_->step(dt, currentTime_);
             currentTime_ += dt;
             currentStep_ = step;
             if (!probePoints_.empty()) pending.push_back({step, currentTime_});
             const bool last =
                 (maxSteps > 0 && step >= maxSteps) || (finalTime > 0.0 && currentTime_ >= finalTime - 1e-13);
             if (step % outFreq == 0 || last) {
+                flush();
                 history_.push_back(solver_->diagnostics());
                 fetchProbes(pending, csv);
             }
         }
+        flush();
         fetchProbes(pending, csv);
         if (csv) std::fclose(csv);
         solver_->downloadState(state_.data(), state_.size());

@@ -4,8 +4,9 @@ preprocess(): box mesh or mesh file, dt rule, initial condition hook evaluated a
 simulate():   DgSolver.initialize + uploadState, then the step loop with the
               reference's termination and last-dt clamp (case.cpp:562-583).
               Unlike the reference (which downloads the whole state every step,
-              case.cpp:588) diagnostics are reduced on the device and the state
-              is downloaded once at the end.
+              case.cpp:588) probes are recorded and diagnostics reduced on the
+              device, the steps between two outputs run as one multi-step
+              launch, and the state is downloaded once at the end.
 postprocess(): user hook, then DgSolver.finalize.
 
 Config keeps the subset of the reference's keys that shape the step
@@ -236,8 +237,18 @@ class Case:
                                                  for q in range(nprobe)) + "\n")
             self.solver.sampleProbes()  # t = 0 (case.cpp:473-474)
             pending.append((0, 0.0))
+        # The reference's step schedule (case.cpp:562-583); the steps between two
+        # outputs with equal dt go to the device as one multi-step launch with
+        # one synchronisation instead of a synchronised step() each.
         step = 0
-        while True:  # case.cpp:562-583
+        chunk = [0.0, 0]  # dt, steps not launched yet
+
+        def flush():
+            if chunk[1]:
+                self.solver.run(chunk[0], chunk[1])
+                chunk[1] = 0
+
+        while True:
             if max_steps > 0 and step >= max_steps:
                 break
             if final_time > 0.0 and self.current_time >= final_time - 1e-13:
@@ -245,17 +256,22 @@ class Case:
             dt = self.dt
             if final_time > 0.0 and self.current_time + dt > final_time:
                 dt = final_time - self.current_time
+            if chunk[1] and dt != chunk[0]:
+                flush()
+            chunk[0] = dt
+            chunk[1] += 1
             step += 1
-            self.solver.step(dt, self.current_time)
             self.current_time += dt
             self.current_step = step
             if nprobe:
                 pending.append((step, self.current_time))
             final = (max_steps > 0 and step >= max_steps) or (final_time > 0.0 and self.current_time >= final_time - 1e-13)
             if step % out_freq == 0 or final:
+                flush()
                 d = self.solver.diagnostics()
                 self.history.append((step, self.current_time, d["max_e"], d["max_h"], d["energy"]))
                 self._fetch_probes(pending, out)
+        flush()
         self._fetch_probes(pending, out)
         if out is not None:
             out.close()

@@ -436,34 +436,42 @@ __global__ void k_l2_final(const double* __restrict__ part, int n, double* __res
     }
 }
 
-// Probe record (ProbeManager::record, probe.cpp:192-216) on the device: one
-// thread per (probe, component) sums w_p u_c over the probe element's nodes in
-// node order, w_p = (Lr[i] Ls[j]) Lt[k] -- the reference's weights and
-// summation order, in FP64 -- into record slot (*slot mod cap) of the history;
-// thread 0 then advances the slot.  One block, so no other block reads the
-// slot while it moves.
+// Probe record (ProbeManager::record, probe.cpp:192-216) on the device, one
+// workgroup per probe: the threads form the rounded products w_p * u_c(p),
+// w_p = (Lr[i] Ls[j]) Lt[k], of all the element's nodes at once into LDS; then
+// one thread per component adds them in node order -- the reference's
+// operations in its order, in FP64, so the record is bit for bit the host's --
+// into record slot (*slot mod cap).  k_probe_advance moves the slot afterwards
+// (stream order), so every workgroup reads the same slot.
 template <typename R>
-__global__ void k_probe(const R* __restrict__ u, long long cs, long long ns, int nprobes,
-                        const long long* __restrict__ base, const double* __restrict__ lw, int n1,
-                        double* __restrict__ hist, int* __restrict__ slot, int cap) {
-    const int rec = *slot % cap;
+__global__ void __launch_bounds__(256) k_probe(const R* __restrict__ u, long long cs, long long ns, int nprobes,
+                                               const long long* __restrict__ base, const double* __restrict__ lw,
+                                               int n1, double* __restrict__ hist, const int* __restrict__ slot,
+                                               int cap) {
+    __shared__ double prod[6][kMaxN1 * kMaxN1 * kMaxN1];
+    const int q = blockIdx.x;
     const int np = n1 * n1 * n1;
-    for (int x = threadIdx.x; x < 6 * nprobes; x += blockDim.x) {
-        const int q = x / 6, c = x - 6 * q;
-        const double* Lr = lw + size_t(q) * 3 * n1;
-        const double* Ls = Lr + n1;
-        const double* Lt = Ls + n1;
-        const R* up = u + base[q] * ns + c * cs;
-        double acc = 0.0;
-        for (int p = 0; p < np; ++p) {
-            const int i = p % n1, j = (p / n1) % n1, kk = p / (n1 * n1);
-            const double w = __dmul_rn(__dmul_rn(Lr[i], Ls[j]), Lt[kk]);
-            acc = __dadd_rn(acc, __dmul_rn(w, double(up[p * ns])));  // no contraction: the host sum's roundings
-        }
-        hist[(size_t(rec) * nprobes + q) * 6 + c] = acc;
+    const double* Lr = lw + size_t(q) * 3 * n1;
+    const double* Ls = Lr + n1;
+    const double* Lt = Ls + n1;
+    const R* up = u + base[q] * ns;
+    for (int p = threadIdx.x; p < np; p += blockDim.x) {
+        const int i = p % n1, j = (p / n1) % n1, kk = p / (n1 * n1);
+        const double w = __dmul_rn(__dmul_rn(Lr[i], Ls[j]), Lt[kk]);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) prod[c][p] = __dmul_rn(w, double(up[p * ns + c * cs]));
     }
     __syncthreads();
-    if (threadIdx.x == 0) *slot = *slot + 1;
+    if (threadIdx.x < 6) {
+        const int c = threadIdx.x;
+        double acc = 0.0;
+        for (int p = 0; p < np; ++p) acc = __dadd_rn(acc, prod[c][p]);  // no contraction: the host sum's roundings
+        hist[(size_t(*slot % cap) * nprobes + q) * 6 + c] = acc;
+    }
+}
+
+__global__ void k_probe_advance(int* p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = *p + 1;
 }
 
 // Sets *slot = 0 with a kernel (no memset node if a caller captures it).
@@ -659,7 +667,11 @@ template <typename R>
 hipError_t launch_probe(const R* u, long long cs, long long ns, int nprobes, const long long* base, const double* lw,
                         int n1, double* hist, int* slot, int cap, hipStream_t s) {
     if (nprobes <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_probe<R>, dim3(1), dim3(256), 0, s, u, cs, ns, nprobes, base, lw, n1, hist, slot, cap);
+    if (n1 < 2 || n1 > kMaxN1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_probe<R>, dim3(nprobes), dim3(256), 0, s, u, cs, ns, nprobes, base, lw, n1, hist, slot, cap);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_probe_advance, dim3(1), dim3(64), 0, s, slot);
     return hipGetLastError();
 }
 template hipError_t launch_probe<double>(const double*, long long, long long, int, const long long*, const double*, int,

@@ -546,7 +546,7 @@ int Solver::set_probes(const Mesh& m, int n, const double* xyz, int capacity) {
     NW_CHECK_ARG(capacity >= 1, "probe history capacity must be >= 1");
     NW_CHECK_ARG(inst.size() == 1, "probes need a single-mesh solver");
     NW_CHECK_ARG(m.n1 == n1 && m.nelt == nelt, "probes: mesh does not match the solver's mesh");
-    NW_CHECK_ARG(6.0 * double(n) <= 256.0 * 64.0, "too many probes (at most 2730)");
+    NW_CHECK_ARG(n <= 65535, "too many probes (at most 65535)");
     NW_TRY_HIP(hipSetDevice(device));
     NW_TRY_HIP(hipStreamSynchronize(stream));
     drop_graphs();

@@ -130,3 +130,33 @@ def test_case_writes_probe_history_csv(tmp_path):
     assert row[0] == "7" and row[1] == "%.8e" % case.probe_times[-1]
     np.testing.assert_allclose([float(v) for v in row[2:]], case.probe_values[-1].reshape(-1), rtol=1e-8, atol=0)
     case.postprocess()
+
+
+def test_case_launches_output_intervals_as_one_run():
+    """Case runs the steps between outputs as one multi-step launch: same step
+    schedule (case.cpp:562-583, including the clamped last dt), same times and
+    the same state bit for bit as one synchronised step() per step."""
+    cfg = Config(order=4, nelx=2, nely=2, nelz=2, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax=1.0,
+                 dt=0.01, num_steps=0, final_time=0.075, output_frequency=3, output_dir="none",
+                 probes=[[0.4, 0.4, 0.6]])
+    case = Case(cfg)
+    case.setInitialConditionHook(lambda x, y, z: (0 * x, 0 * x, np.sin(np.pi * x) * np.sin(np.pi * y), 0 * x,
+                                                  0 * x, 0 * x))
+    case.preprocess()
+    st0 = case.state.copy()
+    case.simulate()
+    s = nw.DgSolver(precision="fp64")
+    s.initialize(case.mesh, cfg.c0)
+    s.uploadState(st0)
+    t, times = 0.0, [0.0]
+    while t < 0.075 - 1e-13:
+        dt = 0.01 if t + 0.01 <= 0.075 else 0.075 - t
+        s.step(dt, t)
+        t += dt
+        times.append(t)
+    assert case.current_step == len(times) - 1 == 8
+    assert case.probe_times == times
+    assert len(case.history) == 3  # steps 3, 6 and the last
+    np.testing.assert_array_equal(case.state, s.downloadState())
+    s.finalize()
+    case.postprocess()
