@@ -248,6 +248,13 @@ void k_fused_stage(FusedArgs A) {
     const int es = tid / NP;
     const int m = tid - es * NP;
     const int i = m % N1, j = (m / N1) % N1, k = m / N2;
+    // FP32 at n1 = 8, one element per workgroup: each wave holds one k-plane, so
+    // whether its nodes lie on a t-face is wave-uniform and the t-face row
+    // writes and t-lift reads of the six interior planes are skipped by a
+    // scalar branch (same arithmetic: the lift still adds the zero row's 0)
+    const bool tface = (N2 == 64 && EPW == 1 && sizeof(R) == 4)
+                           ? (__builtin_amdgcn_readfirstlane(k) == 0 || __builtin_amdgcn_readfirstlane(k) == N1 - 1)
+                           : true;
     const __amdgpu_buffer_rsrc_t rU = make_rsrc(A.uin), rO = make_rsrc(A.uout), rK = make_rsrc(A.kbuf);
     const __amdgpu_buffer_rsrc_t rW = make_rsrc(A.fw), rN = make_rsrc(A.fnb);
     const __amdgpu_buffer_rsrc_t rR = make_rsrc(A.rec);
@@ -454,7 +461,7 @@ void k_fused_stage(FusedArgs A) {
             const R ss = j == 0 ? R(-1) : R(1), sr = i == 0 ? R(-1) : R(1), st = k == 0 ? R(-1) : R(1);
             row(rs, ss * Csx, ss * Csy, ss * Csz);
             row(rr, sr * Crx, sr * Cry, sr * Crz);
-            row(rt, st * Ctx, st * Cty, st * Ctz);
+            if (tface) row(rt, st * Ctx, st * Cty, st * Ctz);
         } else {
             const int b = es * NF;
             if (j == 0) row(b + 0 * N2 + i + N1 * k, -Csx, -Csy, -Csz);
@@ -639,10 +646,18 @@ void k_fused_stage(FusedArgs A) {
             const int sj = j == 0 ? base + 0 * N2 + i + N1 * k : (j == N1 - 1 ? base + 2 * N2 + i + N1 * k : zero);
             const int si = i == N1 - 1 ? base + 1 * N2 + j + N1 * k : (i == 0 ? base + 3 * N2 + j + N1 * k : zero);
             const int sk = k == 0 ? base + 4 * N2 + i + N1 * j : (k == N1 - 1 ? base + 5 * N2 + i + N1 * j : zero);
+            if (tface) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                LE[c] = (sF[sj][c] + sF[si][c]) + sF[sk][c];
-                LH[c] = (sF[sj][c + 3] + sF[si][c + 3]) + sF[sk][c + 3];
+                for (int c = 0; c < 3; ++c) {
+                    LE[c] = (sF[sj][c] + sF[si][c]) + sF[sk][c];
+                    LH[c] = (sF[sj][c + 3] + sF[si][c + 3]) + sF[sk][c + 3];
+                }
+            } else {  // the zero row's value, added as the gather does
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    LE[c] = (sF[sj][c] + sF[si][c]) + R(0);
+                    LH[c] = (sF[sj][c + 3] + sF[si][c + 3]) + R(0);
+                }
             }
         }
 
