@@ -470,8 +470,10 @@ __global__ void __launch_bounds__(256) k_probe(const R* __restrict__ u, long lon
     }
 }
 
-__global__ void k_probe_advance(int* p) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *p = *p + 1;
+// Counts records up to cap + 1 (the overflow the host reports), so a run that
+// never fetches its history cannot wrap the counter.
+__global__ void k_probe_advance(int* p, int cap) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && *p <= cap) *p = *p + 1;
 }
 
 // Sets *slot = 0 with a kernel (no memset node if a caller captures it).
@@ -671,7 +673,7 @@ hipError_t launch_probe(const R* u, long long cs, long long ns, int nprobes, con
     hipLaunchKernelGGL(k_probe<R>, dim3(nprobes), dim3(256), 0, s, u, cs, ns, nprobes, base, lw, n1, hist, slot, cap);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_probe_advance, dim3(1), dim3(64), 0, s, slot);
+    hipLaunchKernelGGL(k_probe_advance, dim3(1), dim3(64), 0, s, slot, cap);
     return hipGetLastError();
 }
 template hipError_t launch_probe<double>(const double*, long long, long long, int, const long long*, const double*, int,
