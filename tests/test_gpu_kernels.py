@@ -1,8 +1,8 @@
 """Kernel-level parity: every reference kernel's counterpart, called through the
 C ABI on oracle-generated inputs, against the oracle's restatement of that kernel
 (dg_kernels.cu:41-453).  Integer/byte movement is bit-exact; FP64 arithmetic
-within the stated tolerance (the reference's own atomics make its last bits
-run-dependent, SURVEY quirk list)."""
+within the stated tolerance (the reference's lift accumulates with FP64
+atomics, dg_kernels.cu:307-335, so its own last bits vary from run to run)."""
 import numpy as np
 import pytest
 

@@ -2,7 +2,7 @@
 FP64 oracle on the same seeded inputs.
 
 Tolerances (relative L2 with the J*w3 weight of 3dboxpec_test.cpp:76-119):
-  * FP64, plain and fused pipelines: <= 1e-12 (BASELINE north star);
+  * FP64, plain and fused pipelines: <= 1e-12;
   * FP32 fields (k FP32 or FP64):    <= 2e-5 after the stated steps;
 and the reference's KATs on the device: energy non-increasing with upwind
 (nekwave_test.cpp:212-293), eigenmode errors (3dboxpec_test.cpp:136).
