@@ -222,3 +222,39 @@ def test_oracle_probe_weights_match_reference_jittered_rea(tmp_path):
     oo, ow = _oracle_weights(o, *o.locate_probes(pts))
     np.testing.assert_array_equal(ro, oo)
     np.testing.assert_array_equal(rw, ow)
+
+
+def test_probe_pins_catch_planted_mistakes():
+    """The exactness pin has teeth: the same record assembled with a plausible
+    mistake (s and t weights swapped, no clamping of (r,s,t), the neighbouring
+    element) misses the polynomial field, while the oracle's record reproduces it."""
+    n1 = 5
+    o = po.Mesh.box(n1, (3, 2, 2), (0.0, 1.5, 0.0, 1.0, -1.0, 1.0))
+    x, y, z = o.x, o.y, o.zc
+    f = x ** 3 * y + z ** 2 * x - y ** 4
+    st = np.concatenate([f] * 6)
+    pts = np.array([[0.1, 0.2, 0.3], [1.2, 0.7, -0.9], [2.0, -1.0, 0.4]])
+    cp = np.clip(pts, [0.0, 0.0, -1.0], [1.5, 1.0, 1.0])
+    want = cp[:, 0] ** 3 * cp[:, 1] + cp[:, 2] ** 2 * cp[:, 0] - cp[:, 1] ** 4
+    elem, rst = o.locate_probes(pts)
+    zz = np.ascontiguousarray(o.z[:n1])
+    np_ = n1 ** 3
+
+    def record(el, r, order=(0, 1, 2)):
+        out = []
+        for q in range(len(pts)):
+            L = []
+            for d in range(3):
+                v = np.zeros(n1)
+                po.lib().or_lagrange_1d(n1, po.dptr(zz), C.c_double(r[q, order[d]]), po.dptr(v))
+                L.append(v)
+            w = (L[0][None, None, :] * L[1][None, :, None]) * L[2][:, None, None]
+            out.append(np.dot(w.reshape(-1), f[el[q] * np_:(el[q] + 1) * np_]))
+        return np.array(out)
+
+    np.testing.assert_allclose(record(elem, rst), want, atol=1e-12)
+    np.testing.assert_allclose(o.probe_record(pts, st)[:, 0], want, atol=1e-12)
+    unclamped = rst.copy()
+    unclamped[2, 0] = 2.0 * (2.0 - 1.0) / 0.5 - 1.0  # r of x = 2.0 in the last x-element, not clamped
+    for bad in (record(elem, rst, (0, 2, 1)), record(elem, unclamped), record(np.minimum(elem + 1, 11), rst)):
+        assert np.max(np.abs(bad - want)) > 1e-3
