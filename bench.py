@@ -243,7 +243,7 @@ def main():
                          "plus a 256 MiB device copy as the HBM byte-count calibration")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (the multi-GPU halo path is design-only); "
+        print("bench.py: nekwave_amd runs on one GPU (the multi-GPU halo path is a design, not code); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
