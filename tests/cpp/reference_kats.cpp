@@ -9,7 +9,9 @@
 // Exit status 0 when every check passes.  Usage: reference_kats [fp64|fp32]
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "nekwave_amd.hpp"
@@ -87,7 +89,9 @@ static void BoxPecCase() {
     cfg.maxSteps = 50;
     cfg.outputFreq = 10;
     cfg.precision = g_prec;
-    cfg.outputDir = "none";
+    char dirTemplate[] = "/tmp/nw_kat_probesXXXXXX";
+    const char* dir = mkdtemp(dirTemplate);
+    cfg.outputDir = dir ? dir : "none";
     cfg.probeRel = {{{0.1, 0.2, -0.1}}};
     c.setConfig(cfg);
     c.addProbe(0.3, -0.4, 0.25);
@@ -133,6 +137,22 @@ static void BoxPecCase() {
     EXPECT_TRUE(c.probeSteps().size() == 51 && c.probeSteps().back() == 50);
     EXPECT_TRUE(c.probeValues().size() == 51 * 2 * 6);
     EXPECT_TRUE(std::isfinite(c.probeValues().back()) && c.probeValues()[5] != 0.0);
+    // probe_history.csv as ProbeManager writes it (probe.cpp:117-133, 192-216)
+    EXPECT_TRUE(dir != nullptr);
+    const std::string csvPath = std::string(dir) + "/probe_history.csv";
+    std::FILE* f = std::fopen(csvPath.c_str(), "r");
+    EXPECT_TRUE(f != nullptr);
+    std::vector<std::string> lines;
+    char buf[4096];
+    while (std::fgets(buf, sizeof(buf), f)) lines.push_back(buf);
+    std::fclose(f);
+    std::remove(csvPath.c_str());
+    std::remove(dir);
+    EXPECT_TRUE(lines.size() == 52);
+    EXPECT_TRUE(lines[0].rfind("step,time,Ex_1,Ey_1,Ez_1,Hx_1,Hy_1,Hz_1,Ex_2,", 0) == 0);
+    char expect[64];
+    std::snprintf(expect, sizeof(expect), "50,%.8e,", c.probeTimes().back());
+    EXPECT_TRUE(lines.back().rfind(expect, 0) == 0);
     std::printf("  PASSED\n");
 }
 
