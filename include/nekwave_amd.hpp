@@ -310,9 +310,10 @@ class Case {
         probeValues_.clear();
         probePoints_ = probeList();
         std::vector<std::pair<int, double>> pending;
+        const int probeCap = std::min(outFreq + 1, kProbeCapacity);
         std::FILE* csv = nullptr;
         if (!probePoints_.empty()) {
-            solver_->setProbes(*mesh_, probePoints_, outFreq + 1);
+            solver_->setProbes(*mesh_, probePoints_, probeCap);
             const std::string& d = config_.outputDir;
             if (!d.empty() && d != "none" && d != "None" && d != "NONE") {  // probe.cpp:43-48, 117-133
                 ::mkdir(d.c_str(), 0755);
@@ -346,7 +347,13 @@ class Case {
             ++step;
             currentTime_ += dt;
             currentStep_ = step;
-            if (!probePoints_.empty()) pending.push_back({step, currentTime_});
+            if (!probePoints_.empty()) {
+                pending.push_back({step, currentTime_});
+                if (static_cast<int>(pending.size()) == probeCap) {  // device history full: fetch now
+                    flush();
+                    fetchProbes(pending, csv);
+                }
+            }
             const bool last =
                 (maxSteps > 0 && step >= maxSteps) || (finalTime > 0.0 && currentTime_ >= finalTime - 1e-13);
             if (step % outFreq == 0 || last) {
@@ -388,6 +395,8 @@ class Case {
     const std::vector<int>& probeSteps() const { return probeSteps_; }
     const std::vector<double>& probeTimes() const { return probeTimes_; }
     const std::vector<double>& probeValues() const { return probeValues_; }
+
+    static constexpr int kProbeCapacity = 4096;  // device probe records kept between fetches
 
    private:
     void fetchProbes(std::vector<std::pair<int, double>>& pending, std::FILE* csv) {

@@ -116,6 +116,8 @@ class Config:
 
 
 class Case:
+    PROBE_CAPACITY = 4096  # device probe records kept between fetches
+
     def __init__(self, config=None):
         self.config = config or Config()
         self.ic = None
@@ -228,7 +230,8 @@ class Case:
         pending, out, self._probe_chunks = [], None, []
         self.probe_steps, self.probe_times = [], []
         if nprobe:
-            self.solver.setProbes(self.mesh, self.probe_points, capacity=out_freq + 1)
+            cap = min(out_freq + 1, self.PROBE_CAPACITY)
+            self.solver.setProbes(self.mesh, self.probe_points, capacity=cap)
             save = c.output_dir and c.output_dir.lower() != "none"
             if save:  # probe.cpp:43-48, 117-133
                 os.makedirs(c.output_dir, exist_ok=True)
@@ -265,6 +268,9 @@ class Case:
             self.current_step = step
             if nprobe:
                 pending.append((step, self.current_time))
+                if len(pending) == cap:  # the device history is full: fetch it before the next step
+                    flush()
+                    self._fetch_probes(pending, out)
             final = (max_steps > 0 and step >= max_steps) or (final_time > 0.0 and self.current_time >= final_time - 1e-13)
             if step % out_freq == 0 or final:
                 flush()

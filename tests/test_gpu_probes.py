@@ -160,3 +160,23 @@ def test_case_launches_output_intervals_as_one_run():
     np.testing.assert_array_equal(case.state, s.downloadState())
     s.finalize()
     case.postprocess()
+
+
+def test_case_fetches_early_when_the_history_fills(monkeypatch):
+    """With a device history shorter than the output interval, Case fetches
+    whenever it fills; the records equal those of a run with room to spare."""
+    def run(cap):
+        monkeypatch.setattr(Case, "PROBE_CAPACITY", cap)
+        cfg = Config(order=4, nelx=2, nely=2, nelz=2, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax=1.0,
+                     num_steps=25, output_frequency=10, output_dir="none", probes=[[0.4, 0.4, 0.6], [0.9, 0.1, 0.5]])
+        case = Case(cfg)
+        case.setInitialConditionHook(lambda x, y, z: (0 * x, 0 * x, np.sin(np.pi * x) * np.sin(np.pi * y), 0 * x,
+                                                      0 * x, 0 * x))
+        case.preprocess()
+        case.simulate()
+        case.postprocess()
+        return case
+    small, big = run(4), run(4096)
+    assert small.probe_steps == big.probe_steps == list(range(26))
+    np.testing.assert_array_equal(small.probe_values, big.probe_values)
+    np.testing.assert_array_equal(small.state, big.state)
