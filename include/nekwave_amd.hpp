@@ -216,6 +216,7 @@ struct Config {
     // the box centre in units of its edge lengths, or num_probes generated ones
     std::vector<std::array<double, 3>> probes, probeRel;
     int numProbes = 0;
+    int numModes = 1;                  // written to probes.json (config "num_modes")
     std::string outputDir = "output";  // "none" keeps probe records in memory only
 };
 
@@ -323,6 +324,7 @@ class Case {
                     for (size_t q = 1; q <= probePoints_.size(); ++q)
                         std::fprintf(csv, ",Ex_%zu,Ey_%zu,Ez_%zu,Hx_%zu,Hy_%zu,Hz_%zu", q, q, q, q, q, q);
                     std::fputs("\n", csv);
+                    writeProbesJson(d + "/probes.json");
                 }
             }
             solver_->sampleProbes();  // t = 0 (case.cpp:473-474)
@@ -399,6 +401,44 @@ class Case {
     static constexpr int kProbeCapacity = 4096;  // device probe records kept between fetches
 
    private:
+    // probes.json as ProbeManager::init writes it (probe.cpp:135-173): placed by
+    // interpolation, so nodeIdx -1, actual = target, offset 0; the stream turns
+    // scientific (precision 8) at the separation line, which exists only for two
+    // or more probes, so a single probe prints in the default format.
+    void writeProbesJson(const std::string& path) const {
+        std::FILE* f = std::fopen(path.c_str(), "w");
+        if (!f) return;
+        const auto& p = probePoints_;
+        const bool sci = p.size() >= 2;
+        const char* num = sci ? "%.8e" : "%g";
+        std::fprintf(f, "{\n  \"order\": %d,\n  \"num_modes\": %d,\n  \"num_probes\": %zu,\n", mesh_->getN(),
+                     config_.numModes, p.size());
+        if (sci) {
+            const double dx = p[1][0] - p[0][0], dy = p[1][1] - p[0][1], dz = p[1][2] - p[0][2];
+            const double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+            std::fprintf(f,
+                         "  \"separation_distance\": %.8e,\n  \"delta_x\": %.8e,\n  \"delta_y\": %.8e,\n"
+                         "  \"delta_z\": %.8e,\n  \"domain_Lx\": %.8e,\n  \"elements_x\": %d,\n",
+                         dist, dx, dy, dz, config_.xmax - config_.xmin, config_.nelx);
+        }
+        std::fputs("  \"probes\": [\n", f);
+        for (size_t q = 0; q < p.size(); ++q) {
+            std::fprintf(f, "    {\n      \"id\": %zu,\n      \"nodeIdx\": -1,\n", q + 1);
+            for (const char* key : {"target", "actual"}) {
+                std::fprintf(f, "      \"%s\": [", key);
+                for (int d = 0; d < 3; ++d) {
+                    std::fprintf(f, num, p[q][d]);
+                    std::fputs(d < 2 ? ", " : "],\n", f);
+                }
+            }
+            std::fputs("      \"offset\": ", f);
+            std::fprintf(f, num, 0.0);
+            std::fprintf(f, "\n    }%s\n", q + 1 < p.size() ? "," : "");
+        }
+        std::fputs("  ]\n}\n", f);
+        std::fclose(f);
+    }
+
     void fetchProbes(std::vector<std::pair<int, double>>& pending, std::FILE* csv) {
         if (pending.empty()) return;
         int n = 0;

@@ -17,7 +17,8 @@ jitter, seed.
 
 Probes (probe.cpp:33-216, case.cpp:381-424, 473-474, 594-596) are recorded on the
 device every step; the small history comes back at output frequency and is
-written to <output_dir>/probe_history.csv in the reference's format.
+written to <output_dir>/probe_history.csv in the reference's format, beside
+probes.json.
 """
 import json
 import math
@@ -33,6 +34,32 @@ def _six_sig(v):
     """The reference round-trips JSON numbers through a default-precision string
     (json.hpp:52-61), keeping 6 significant digits."""
     return float("%g" % float(v))
+
+
+def probes_json_text(order, num_modes, points, domain_lx, elements_x, mode_k=()):
+    """probes.json as ProbeManager::init writes it (probe.cpp:135-173). The probe
+    is placed by interpolation, so nodeIdx is -1, actual = target and offset 0.
+    The stream turns std::scientific with precision 8 at the separation line,
+    which exists only for two or more probes: with one probe the numbers print
+    in the default (%g) format."""
+    pts = [tuple(float(v) for v in p) for p in points]
+    sci = len(pts) >= 2
+    num = (lambda v: "%.8e" % v) if sci else (lambda v: "%g" % v)
+    out = ["{", '  "order": %d,' % order, '  "num_modes": %d,' % num_modes, '  "num_probes": %d,' % len(pts)]
+    if sci:
+        dx, dy, dz = pts[1][0] - pts[0][0], pts[1][1] - pts[0][1], pts[1][2] - pts[0][2]
+        dist = math.sqrt(dx * dx + dy * dy + dz * dz)
+        out += ['  "separation_distance": %.8e,' % dist, '  "delta_x": %.8e,' % dx, '  "delta_y": %.8e,' % dy,
+                '  "delta_z": %.8e,' % dz, '  "domain_Lx": %.8e,' % domain_lx, '  "elements_x": %d,' % elements_x]
+        if len(mode_k):
+            out.append('  "k_modes": [' + ", ".join("%.8e" % k for k in mode_k) + "],")
+    out.append('  "probes": [')
+    for q, p in enumerate(pts):
+        xyz = ", ".join(num(v) for v in p)
+        out += ["    {", '      "id": %d,' % (q + 1), '      "nodeIdx": -1,', '      "target": [%s],' % xyz,
+                '      "actual": [%s],' % xyz, '      "offset": %s' % num(0.0), "    }" + ("," if q + 1 < len(pts) else "")]
+    out += ["  ]", "}"]
+    return "\n".join(out) + "\n"
 
 
 class Config:
@@ -62,6 +89,7 @@ class Config:
         self.probes = []
         self.relative_probes = []
         self.num_probes = 0
+        self.num_modes = 1
         self.output_dir = "output"
         for k, v in kw.items():
             self.set(k, v)
@@ -238,6 +266,9 @@ class Case:
                 out = open(os.path.join(c.output_dir, "probe_history.csv"), "w")
                 out.write("step,time" + "".join(",Ex_%d,Ey_%d,Ez_%d,Hx_%d,Hy_%d,Hz_%d" % ((q + 1,) * 6)
                                                  for q in range(nprobe)) + "\n")
+                with open(os.path.join(c.output_dir, "probes.json"), "w") as fh:
+                    fh.write(probes_json_text(self.mesh.getN(), c.num_modes, self.probe_points, c.xmax - c.xmin,
+                                              c.nelx))
             self.solver.sampleProbes()  # t = 0 (case.cpp:473-474)
             pending.append((0, 0.0))
         # The reference's step schedule (case.cpp:562-583); the steps between two

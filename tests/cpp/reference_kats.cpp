@@ -147,7 +147,16 @@ static void BoxPecCase() {
     while (std::fgets(buf, sizeof(buf), f)) lines.push_back(buf);
     std::fclose(f);
     std::remove(csvPath.c_str());
+    const std::string jsonPath = std::string(dir) + "/probes.json";
+    std::FILE* jf = std::fopen(jsonPath.c_str(), "r");
+    EXPECT_TRUE(jf != nullptr);
+    std::vector<std::string> jl;
+    while (std::fgets(buf, sizeof(buf), jf)) jl.push_back(buf);
+    std::fclose(jf);
+    std::remove(jsonPath.c_str());
     std::remove(dir);
+    EXPECT_TRUE(jl.size() > 10 && jl[1] == "  \"order\": 4,\n" && jl[3] == "  \"num_probes\": 2,\n");
+    EXPECT_TRUE(jl[4].rfind("  \"separation_distance\": ", 0) == 0);
     EXPECT_TRUE(lines.size() == 52);
     EXPECT_TRUE(lines[0].rfind("step,time,Ex_1,Ey_1,Ez_1,Hx_1,Hy_1,Hz_1,Ex_2,", 0) == 0);
     char expect[64];

@@ -129,6 +129,10 @@ def test_case_writes_probe_history_csv(tmp_path):
     row = lines[-1].split(",")
     assert row[0] == "7" and row[1] == "%.8e" % case.probe_times[-1]
     np.testing.assert_allclose([float(v) for v in row[2:]], case.probe_values[-1].reshape(-1), rtol=1e-8, atol=0)
+    import json
+    meta = json.load(open(os.path.join(out_dir, "probes.json")))
+    assert meta["order"] == 5 and meta["num_probes"] == 2 and meta["elements_x"] == 2
+    np.testing.assert_allclose([p["target"] for p in meta["probes"]], case.probe_points, rtol=1e-8)
     case.postprocess()
 
 

@@ -258,3 +258,29 @@ def test_probe_pins_catch_planted_mistakes():
     unclamped[2, 0] = 2.0 * (2.0 - 1.0) / 0.5 - 1.0  # r of x = 2.0 in the last x-element, not clamped
     for bad in (record(elem, rst, (0, 2, 1)), record(elem, unclamped), record(np.minimum(elem + 1, 11), rst)):
         assert np.max(np.abs(bad - want)) > 1e-3
+
+
+def test_probes_json_format():
+    """probes.json as ProbeManager::init writes it (probe.cpp:135-173), including
+    the stream's switch to scientific notation only when two or more probes exist."""
+    from nekwave_amd.case import probes_json_text
+    one = probes_json_text(5, 1, [[0.3, 0.6, 0.2]], 1.0, 2)
+    assert one == ('{\n  "order": 5,\n  "num_modes": 1,\n  "num_probes": 1,\n  "probes": [\n    {\n'
+                   '      "id": 1,\n      "nodeIdx": -1,\n      "target": [0.3, 0.6, 0.2],\n'
+                   '      "actual": [0.3, 0.6, 0.2],\n      "offset": 0\n    }\n  ]\n}\n')
+    two = probes_json_text(8, 3, [[0.25, 0.5, 0.5], [0.75, 0.5, 0.5]], 1.0, 2, mode_k=[3.0])
+    assert two == ('{\n  "order": 8,\n  "num_modes": 3,\n  "num_probes": 2,\n'
+                   '  "separation_distance": 5.00000000e-01,\n  "delta_x": 5.00000000e-01,\n'
+                   '  "delta_y": 0.00000000e+00,\n  "delta_z": 0.00000000e+00,\n'
+                   '  "domain_Lx": 1.00000000e+00,\n  "elements_x": 2,\n  "k_modes": [3.00000000e+00],\n'
+                   '  "probes": [\n'
+                   '    {\n      "id": 1,\n      "nodeIdx": -1,\n'
+                   '      "target": [2.50000000e-01, 5.00000000e-01, 5.00000000e-01],\n'
+                   '      "actual": [2.50000000e-01, 5.00000000e-01, 5.00000000e-01],\n'
+                   '      "offset": 0.00000000e+00\n    },\n'
+                   '    {\n      "id": 2,\n      "nodeIdx": -1,\n'
+                   '      "target": [7.50000000e-01, 5.00000000e-01, 5.00000000e-01],\n'
+                   '      "actual": [7.50000000e-01, 5.00000000e-01, 5.00000000e-01],\n'
+                   '      "offset": 0.00000000e+00\n    }\n  ]\n}\n')
+    import json
+    assert json.loads(two)["separation_distance"] == 0.5
