@@ -216,7 +216,13 @@ struct Config {
     // the box centre in units of its edge lengths, or num_probes generated ones
     std::vector<std::array<double, 3>> probes, probeRel;
     int numProbes = 0;
-    int numModes = 1;                  // written to probes.json (config "num_modes")
+    int numModes = 1;                  // config "num_modes" (probes.json; modes of the bloch wave)
+    // built-in initial conditions (case.cpp:247-372); NaN = the reference's default
+    double pulseSigma = 20.0;
+    double carrierK = std::nan("");
+    double angleDeg = 0.0, kStart = 0.0, kEnd = 0.0;
+    double packetSigma = 0.30;
+    double packetX0 = std::nan("");
     std::string outputDir = "output";  // "none" keeps probe records in memory only
 };
 
@@ -232,6 +238,11 @@ class Case {
     void setInitialCondition(InitialConditionFn fn) { ic_ = std::move(fn); }
     void setInitialConditionHook(InitialConditionFn fn) { setInitialCondition(std::move(fn)); }
     void setPostprocessingHook(PostprocessingFn fn) { post_ = std::move(fn); }
+    // Built-in initial condition used when no hook is set (case.cpp:59-62, 164-176,
+    // 247-372): "gaussian", "bloch"/"periodic", "multimode", "packet"; "3dboxper"
+    // builds a [-1, 1]^3 box on [0, 2 pi]^3 instead (Mesh::rescale, box meshes).
+    void setWaveType(const std::string& w) { waveType_ = w; }
+    const std::string& waveType() const { return waveType_; }
     void addProbe(double x, double y, double z) { customProbes_.push_back({{x, y, z}}); }
     void addRelativeProbe(double rx, double ry, double rz) { customRel_.push_back({{rx, ry, rz}}); }
 
@@ -262,12 +273,17 @@ class Case {
     // case.cpp:98-524 (box meshes; dt rule :196-210; IC hook :227-246)
     void preprocess() {
         mesh_.reset(new Mesh(config_.order));
-        if (config_.nelx > 0 && config_.nely > 0 && config_.nelz > 0)
-            mesh_->createBoxMesh(config_.nelx, config_.nely, config_.nelz, config_.xmin, config_.xmax, config_.ymin,
-                                 config_.ymax, config_.zmin, config_.zmax, config_.periodicX, config_.periodicY,
-                                 config_.periodicZ, config_.jitter, config_.seed);
+        const bool boxGiven = config_.nelx > 0 && config_.nely > 0 && config_.nelz > 0;
+        double b[6] = {config_.xmin, config_.xmax, config_.ymin, config_.ymax, config_.zmin, config_.zmax};
+        if (!boxGiven) b[0] = b[2] = b[4] = -1.0, b[1] = b[3] = b[5] = 1.0;
+        if (waveType_ == "3dboxper" && std::abs(b[0] + 1.0) < 1e-3 && std::abs(b[1] - 1.0) < 1e-3)
+            b[0] = b[2] = b[4] = 0.0, b[1] = b[3] = b[5] = 2.0 * M_PI;  // case.cpp:165-176
+        if (boxGiven)
+            mesh_->createBoxMesh(config_.nelx, config_.nely, config_.nelz, b[0], b[1], b[2], b[3], b[4], b[5],
+                                 config_.periodicX, config_.periodicY, config_.periodicZ, config_.jitter,
+                                 config_.seed);
         else
-            mesh_->createBoxMesh(3, 3, 3, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0);
+            mesh_->createBoxMesh(3, 3, 3, b[0], b[1], b[2], b[3], b[4], b[5]);
         const double dxmin = mesh_->computeMinNodeDistance();
         if (config_.dt > 0.0) {
             dt_ = config_.dt;
@@ -281,11 +297,15 @@ class Case {
         }
         const int npts = mesh_->getTotalPoints();
         state_.assign(6 * size_t(npts), 0.0);
+        numModesUsed_ = config_.numModes;
+        modeK_.clear();
         if (ic_) {
             const auto x = mesh_->getCoordX(), y = mesh_->getCoordY(), z = mesh_->getCoordZ();
             for (int k = 0; k < npts; ++k)
                 ic_(x[k], y[k], z[k], state_[k], state_[npts + k], state_[2 * size_t(npts) + k],
                     state_[3 * size_t(npts) + k], state_[4 * size_t(npts) + k], state_[5 * size_t(npts) + k]);
+        } else if (!waveType_.empty()) {
+            builtinIc();
         }
         currentTime_ = 0.0;
         currentStep_ = 0;
@@ -401,6 +421,67 @@ class Case {
     static constexpr int kProbeCapacity = 4096;  // device probe records kept between fetches
 
    private:
+    // The reference's built-in fields at the nodes (case.cpp:247-372).
+    void builtinIc() {
+        const Config& c = config_;
+        const auto x = mesh_->getCoordX(), y = mesh_->getCoordY(), z = mesh_->getCoordZ();
+        const size_t n = x.size();
+        double *Ez = &state_[2 * n], *Hx = &state_[3 * n], *Hy = &state_[4 * n];
+        const double lx = c.xmax - c.xmin, ly = c.ymax - c.ymin;
+        if (waveType_ == "bloch" || waveType_ == "periodic") {
+            const double a = c.angleDeg, rad = a * M_PI / 180.0;
+            double kx, ky;
+            if (std::abs(a - 0.0) < 1e-4) kx = 2.0 * M_PI / lx, ky = 0.0;
+            else if (std::abs(a - 45.0) < 1e-4) kx = 2.0 * M_PI / lx, ky = 2.0 * M_PI / ly;
+            else if (std::abs(a - 90.0) < 1e-4) kx = 0.0, ky = 2.0 * M_PI / ly;
+            else if (std::abs(a - 26.6) < 1.0 || std::abs(a - 30.0) < 5.0) kx = 4.0 * M_PI / lx, ky = 2.0 * M_PI / ly;
+            else if (std::abs(a - 18.4) < 1.0 || std::abs(a - 15.0) < 5.0) kx = 6.0 * M_PI / lx, ky = 2.0 * M_PI / ly;
+            else {
+                const int mx = std::max(1, (int)std::round(std::cos(rad) * 4.0));
+                const int my = (int)std::round(std::sin(rad) * 4.0);
+                kx = 2.0 * M_PI * mx / lx, ky = 2.0 * M_PI * my / ly;
+            }
+            const double kMag = std::sqrt(kx * kx + ky * ky), ca = kx / kMag, sa = ky / kMag;
+            const int nm = std::min(c.numModes, std::max(1, c.nelx * (mesh_->getN() - 1) / 2));
+            if (c.kStart > 0.0 && c.kEnd > 0.0)
+                for (int m = 0; m < nm; ++m) modeK_.push_back(nm > 1 ? c.kStart + m * (c.kEnd - c.kStart) / (nm - 1) : c.kStart);
+            else
+                for (int m = 1; m <= nm; ++m) modeK_.push_back(m * kMag);
+            const bool carrier = !std::isnan(c.carrierK);
+            const double ck = carrier ? c.carrierK : 2.0 * M_PI / lx;
+            for (size_t k = 0; k < n; ++k) {
+                const double xt = x[k] - c.xmin, yt = y[k] - c.ymin;
+                double e = 0.0;
+                if (nm > 1) {
+                    for (double km : modeK_) e += std::cos(km * (ca * xt + sa * yt));
+                    e /= std::sqrt((double)nm);
+                } else {
+                    e = std::cos((carrier && std::abs(a) < 1e-4) ? ck * xt : kx * xt + ky * yt);
+                }
+                Ez[k] = e;
+                Hx[k] = sa * e;
+                Hy[k] = -ca * e;
+            }
+            numModesUsed_ = nm;
+        } else if (waveType_ == "multimode") {
+            for (size_t k = 0; k < n; ++k) {
+                double e = 0.0;
+                for (int m = 1; m <= 8; ++m) e += (1.0 / m) * std::sin(m * M_PI / lx * (x[k] - c.xmin));
+                Ez[k] = e * std::sin(M_PI * (y[k] - c.ymin) / ly);
+            }
+        } else if (waveType_ == "packet") {
+            const double ck = std::isnan(c.carrierK) ? 4.0 * M_PI : c.carrierK;
+            const double x0 = std::isnan(c.packetX0) ? c.xmin + 0.25 * lx : c.packetX0;
+            for (size_t k = 0; k < n; ++k) {
+                const double d = x[k] - x0;
+                Ez[k] = std::exp(-(d * d) / (2.0 * c.packetSigma * c.packetSigma)) * std::sin(ck * (x[k] - c.xmin)) *
+                        std::sin(M_PI * (y[k] - c.ymin) / ly);
+            }
+        } else if (waveType_ == "gaussian") {
+            for (size_t k = 0; k < n; ++k) Ez[k] = std::exp(-c.pulseSigma * (x[k] * x[k] + y[k] * y[k] + z[k] * z[k]));
+        }
+    }
+
     // probes.json as ProbeManager::init writes it (probe.cpp:135-173): placed by
     // interpolation, so nodeIdx -1, actual = target, offset 0; the stream turns
     // scientific (precision 8) at the separation line, which exists only for two
@@ -412,7 +493,7 @@ class Case {
         const bool sci = p.size() >= 2;
         const char* num = sci ? "%.8e" : "%g";
         std::fprintf(f, "{\n  \"order\": %d,\n  \"num_modes\": %d,\n  \"num_probes\": %zu,\n", mesh_->getN(),
-                     config_.numModes, p.size());
+                     numModesUsed_, p.size());
         if (sci) {
             const double dx = p[1][0] - p[0][0], dy = p[1][1] - p[0][1], dz = p[1][2] - p[0][2];
             const double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
@@ -420,6 +501,12 @@ class Case {
                          "  \"separation_distance\": %.8e,\n  \"delta_x\": %.8e,\n  \"delta_y\": %.8e,\n"
                          "  \"delta_z\": %.8e,\n  \"domain_Lx\": %.8e,\n  \"elements_x\": %d,\n",
                          dist, dx, dy, dz, config_.xmax - config_.xmin, config_.nelx);
+            if (!modeK_.empty()) {
+                std::fputs("  \"k_modes\": [", f);
+                for (size_t m = 0; m < modeK_.size(); ++m)
+                    std::fprintf(f, "%.8e%s", modeK_[m], m + 1 < modeK_.size() ? ", " : "");
+                std::fputs("],\n", f);
+            }
         }
         std::fputs("  \"probes\": [\n", f);
         for (size_t q = 0; q < p.size(); ++q) {
@@ -467,6 +554,9 @@ class Case {
     std::vector<nw_diag> history_;
     std::vector<std::array<double, 3>> customProbes_, customRel_, probePoints_;
     std::vector<int> probeSteps_;
+    std::string waveType_;
+    int numModesUsed_ = 1;
+    std::vector<double> modeK_;
     std::vector<double> probeTimes_, probeValues_;
     double dt_ = 0.0, cfl_ = 0.0, currentTime_ = 0.0;
     int currentStep_ = 0;

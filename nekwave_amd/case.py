@@ -36,6 +36,11 @@ def _six_sig(v):
     return float("%g" % float(v))
 
 
+def _round_half_away(v):
+    """std::round: halves away from zero."""
+    return math.floor(v + 0.5) if v >= 0 else -math.floor(-v + 0.5)
+
+
 def probes_json_text(order, num_modes, points, domain_lx, elements_x, mode_k=()):
     """probes.json as ProbeManager::init writes it (probe.cpp:135-173). The probe
     is placed by interpolation, so nodeIdx is -1, actual = target and offset 0.
@@ -68,7 +73,7 @@ class Config:
                 "output_frequency": "output_frequency", "outputfreq": "output_frequency",
                 "flux": "flux_type", "mesh": "mesh_file", "file": "mesh_file", "path": "mesh_file",
                 "meshfile": "mesh_file", "probe": "probes", "probe_rel": "relative_probes",
-                "outputdir": "output_dir"}
+                "outputdir": "output_dir", "sigma": "pulse_sigma"}
 
     def __init__(self, **kw):
         self.order = 3
@@ -90,6 +95,14 @@ class Config:
         self.relative_probes = []
         self.num_probes = 0
         self.num_modes = 1
+        # built-in initial conditions (case.cpp:247-372; the wave type is set on the Case)
+        self.pulse_sigma = 20.0
+        self.carrier_k = None
+        self.angle_deg = 0.0
+        self.k_start = 0.0
+        self.k_end = 0.0
+        self.packet_sigma = 0.30
+        self.packet_x0 = None
         self.output_dir = "output"
         for k, v in kw.items():
             self.set(k, v)
@@ -114,7 +127,9 @@ class Config:
         if not hasattr(self, k):
             raise KeyError("unknown config key %r" % key)
         cur = getattr(self, k)
-        if isinstance(cur, bool):
+        if cur is None:
+            val = float(val)
+        elif isinstance(cur, bool):
             val = str(val).lower() in ("1", "true", "yes", "on") if not isinstance(val, bool) else val
         elif isinstance(cur, int) and not isinstance(cur, bool):
             val = int(val)
@@ -160,10 +175,22 @@ class Case:
         self.history = []
         self.custom_probes = []
         self.custom_relative_probes = []
+        self.wave_type = ""
+        self.num_modes = 1  # as used (the bloch wave clamps it), written to probes.json
+        self.mode_k = []
         self.probe_points = np.zeros((0, 3))
         self.probe_steps = []
         self.probe_times = []
         self.probe_values = np.zeros((0, 0, 6))
+
+    def setWaveType(self, wave_type):
+        """Built-in initial condition when no hook is set (case.cpp:59-62, 164-176,
+        247-372): "gaussian", "bloch"/"periodic", "multimode", "packet"; "3dboxper"
+        maps a [-1, 1]^3 mesh onto [0, 2 pi]^3 (Mesh::rescale, mesh.cpp:639-677)."""
+        self.wave_type = str(wave_type)
+
+    def waveType(self):
+        return self.wave_type
 
     def addProbe(self, x, y, z):
         self.custom_probes.append((float(x), float(y), float(z)))
@@ -227,6 +254,14 @@ class Case:
                 self.mesh.createBoxMesh(3, 3, 3, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0)
         else:  # the reference's default 3x3x3 box (case.cpp:158-161)
             self.mesh.createBoxMesh(3, 3, 3, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0)
+        if self.wave_type == "3dboxper":  # case.cpp:165-176
+            corners = self.mesh.getElementCorners()
+            if abs(corners[..., 0].min() + 1.0) < 1e-3 and abs(corners[..., 0].max() - 1.0) < 1e-3:
+                lo, hi = corners.reshape(-1, 3).min(0), corners.reshape(-1, 3).max(0)
+                span = np.where(hi - lo < 1e-14, 1.0, hi - lo)
+                scaled = 0.0 + (corners - lo) / span * (2.0 * math.pi - 0.0)
+                kind, ne, nf = self.mesh.getFaces()
+                self.mesh = Mesh.fromCorners(c.order, scaled, kind, ne, nf)
         dxmin = self.mesh.computeMinNodeDistance()
         if c.dt > 0.0:  # case.cpp:201-210
             self.dt = c.dt
@@ -238,13 +273,70 @@ class Case:
             self.cfl = Mesh.computeAutomaticCFL(0.4)
             self.dt = self.mesh.computeAutomaticDt(1.0, 0.4)
         x, y, z = self.mesh.coords()
+        self.num_modes, self.mode_k = c.num_modes, []
         if self.ic is not None:
             f = self.ic(x, y, z)
             self.state = np.concatenate([np.broadcast_to(np.asarray(v, dtype=np.float64), x.shape) for v in f])
         else:
             self.state = np.zeros(6 * x.size)
+            self._builtin_ic(x, y, z)
         self.current_time = 0.0
         self.current_step = 0
+
+    def _builtin_ic(self, x, y, z):
+        """The reference's built-in fields (case.cpp:247-372), evaluated at the nodes."""
+        c, n = self.config, x.size
+        st = self.state
+        lx, ly = c.xmax - c.xmin, c.ymax - c.ymin
+        xt, yt = x - c.xmin, y - c.ymin
+        w = self.wave_type
+        if w in ("bloch", "periodic"):
+            ang = c.angle_deg
+            rad = ang * math.pi / 180.0
+            if abs(ang - 0.0) < 1e-4:
+                kx, ky = 2.0 * math.pi / lx, 0.0
+            elif abs(ang - 45.0) < 1e-4:
+                kx, ky = 2.0 * math.pi / lx, 2.0 * math.pi / ly
+            elif abs(ang - 90.0) < 1e-4:
+                kx, ky = 0.0, 2.0 * math.pi / ly
+            elif abs(ang - 26.6) < 1.0 or abs(ang - 30.0) < 5.0:
+                kx, ky = 4.0 * math.pi / lx, 2.0 * math.pi / ly
+            elif abs(ang - 18.4) < 1.0 or abs(ang - 15.0) < 5.0:
+                kx, ky = 6.0 * math.pi / lx, 2.0 * math.pi / ly
+            else:
+                mx = max(1, int(_round_half_away(math.cos(rad) * 4.0)))
+                my = int(_round_half_away(math.sin(rad) * 4.0))
+                kx, ky = 2.0 * math.pi * mx / lx, 2.0 * math.pi * my / ly
+            kmag = math.sqrt(kx * kx + ky * ky)
+            ca, sa = kx / kmag, ky / kmag
+            max_modes = max(1, (c.nelx * (self.mesh.getN() - 1)) // 2)
+            nm = min(c.num_modes, max_modes)
+            if c.k_start > 0.0 and c.k_end > 0.0:
+                ks = [c.k_start + m * (c.k_end - c.k_start) / (nm - 1) if nm > 1 else c.k_start for m in range(nm)]
+            else:
+                ks = [m * kmag for m in range(1, nm + 1)]
+            if nm > 1:
+                e = np.zeros(n)
+                for k in ks:
+                    e += np.cos(k * (ca * xt + sa * yt))
+                e /= math.sqrt(nm)
+            else:
+                carrier = c.carrier_k if c.carrier_k is not None else 2.0 * math.pi / lx
+                e = np.cos(carrier * xt if (c.carrier_k is not None and abs(ang) < 1e-4) else kx * xt + ky * yt)
+            st[2 * n:3 * n], st[3 * n:4 * n], st[4 * n:5 * n] = e, sa * e, -ca * e
+            self.num_modes, self.mode_k = nm, ks
+        elif w == "multimode":
+            e = np.zeros(n)
+            for m in range(1, 9):
+                e += (1.0 / m) * np.sin(m * math.pi / lx * xt)
+            st[2 * n:3 * n] = e * np.sin(math.pi * yt / ly)
+        elif w == "packet":
+            carrier = c.carrier_k if c.carrier_k is not None else 4.0 * math.pi
+            x0 = c.packet_x0 if c.packet_x0 is not None else c.xmin + 0.25 * lx
+            env = np.exp(-((x - x0) ** 2) / (2.0 * c.packet_sigma * c.packet_sigma))
+            st[2 * n:3 * n] = env * np.sin(carrier * (x - c.xmin)) * np.sin(math.pi * yt / ly)
+        elif w == "gaussian":
+            st[2 * n:3 * n] = np.exp(-c.pulse_sigma * (x * x + y * y + z * z))
 
     def simulate(self):
         c = self.config
@@ -267,8 +359,8 @@ class Case:
                 out.write("step,time" + "".join(",Ex_%d,Ey_%d,Ez_%d,Hx_%d,Hy_%d,Hz_%d" % ((q + 1,) * 6)
                                                  for q in range(nprobe)) + "\n")
                 with open(os.path.join(c.output_dir, "probes.json"), "w") as fh:
-                    fh.write(probes_json_text(self.mesh.getN(), c.num_modes, self.probe_points, c.xmax - c.xmin,
-                                              c.nelx))
+                    fh.write(probes_json_text(self.mesh.getN(), self.num_modes, self.probe_points, c.xmax - c.xmin,
+                                              c.nelx, self.mode_k))
             self.solver.sampleProbes()  # t = 0 (case.cpp:473-474)
             pending.append((0, 0.0))
         # The reference's step schedule (case.cpp:562-583); the steps between two

@@ -165,10 +165,41 @@ static void BoxPecCase() {
     std::printf("  PASSED\n");
 }
 
+// Built-in Bloch wave (case.cpp:247-329): a plane wave (|H| = |E|), modes clamped
+// to the grid's Nyquist limit, every cosine 1 at the box's corner line.
+static void BuiltinBlochIc() {
+    std::printf("[RUN] built-in bloch initial condition\n");
+    Case c;
+    Config cfg;
+    cfg.order = 5;
+    cfg.nelx = 4;
+    cfg.nely = cfg.nelz = 2;
+    cfg.xmin = 0.0, cfg.xmax = 2.0, cfg.ymin = 0.0, cfg.ymax = 1.0, cfg.zmin = 0.0, cfg.zmax = 1.0;
+    cfg.numModes = 50;
+    c.setConfig(cfg);
+    c.setWaveType("bloch");
+    c.preprocess();
+    const auto& st = c.state();
+    const size_t n = st.size() / 6;
+    const auto x = c.mesh().getCoordX(), y = c.mesh().getCoordY();
+    int corner = 0;
+    for (size_t k = 0; k < n; ++k) {
+        const double ez = st[2 * n + k], hx = st[3 * n + k], hy = st[4 * n + k];
+        EXPECT_TRUE(std::abs(hx * hx + hy * hy - ez * ez) < 1e-13);
+        if (x[k] == 0.0 && y[k] == 0.0) {
+            EXPECT_TRUE(std::abs(ez - std::sqrt(8.0)) < 1e-13);  // 8 = 4 elements * 4 / 2 modes
+            ++corner;
+        }
+    }
+    EXPECT_TRUE(corner > 0);
+    std::printf("  PASSED\n");
+}
+
 int main(int argc, char** argv) {
     if (argc > 1 && std::strcmp(argv[1], "fp32") == 0) g_prec = NW_PREC_FP32;
     try {
         MaxwellStabilityTest();
+        BuiltinBlochIc();
         BoxPecCase();
     } catch (const Error& e) {
         std::fprintf(stderr, "error %d: %s\n", e.status(), e.what());
