@@ -184,3 +184,31 @@ def test_case_fetches_early_when_the_history_fills(monkeypatch):
     assert small.probe_steps == big.probe_steps == list(range(26))
     np.testing.assert_array_equal(small.probe_values, big.probe_values)
     np.testing.assert_array_equal(small.state, big.state)
+
+
+def test_probe_entry_points_reject_bad_calls():
+    """Error paths of nw_dg_set_probes / nw_dg_probe_sample: a batch of several
+    meshes, a mesh that is not the solver's, a zero capacity and sampling before
+    any probe is placed come back as status codes."""
+    import ctypes as C
+
+    from nekwave_amd import _lib
+
+    m, o = pair(4, (2, 2, 2))
+    m2, _ = pair(4, (3, 2, 2))
+    xyz = np.ascontiguousarray(PTS[:1])
+    ptr = xyz.ctypes.data_as(C.POINTER(C.c_double))
+    b = nw.BatchSolver(precision="fp64")
+    b.initialize([m, m2], 1.0)
+    assert nw.lib().nw_dg_set_probes(b._h, m.handle, 1, ptr, 4) == _lib.NW_ERR_ARG
+    b.finalize()
+    s = nw.DgSolver(precision="fp64")
+    s.initialize(m, 1.0)
+    s.uploadState(random_state(o, 1))
+    assert nw.lib().nw_dg_probe_sample(s._handle()) == _lib.NW_ERR_STATE
+    assert nw.lib().nw_dg_set_probes(s._handle(), m2.handle, 1, ptr, 4) == _lib.NW_ERR_ARG
+    assert nw.lib().nw_dg_set_probes(s._handle(), m.handle, 1, ptr, 0) == _lib.NW_ERR_ARG
+    s.setProbes(m, xyz, capacity=2)  # and the solver still works afterwards
+    s.sampleProbes()
+    assert s.probeHistory().shape == (1, 1, 6)
+    s.finalize()
