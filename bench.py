@@ -56,28 +56,19 @@ def make_solver(nw, wl, precision, pipeline="fused"):
     return s
 
 
-def reload(s, wl):
-    if len(wl.meshes) == 1:
-        s.uploadState(wl.states[0])
-    else:
-        for x, st in enumerate(wl.states):
-            s.uploadState(x, st)
-
-
-def time_path(nw, wl, precision, pipeline, steps, warmup):
+def time_path(nw, wl, precision, pipeline, steps, warmup, repeats=5):
+    """`warmup` untimed steps, then `repeats` timings of exactly `steps` steps,
+    queued back to back on the solver's stream (HIP events around each; no host
+    upload or wait between the warm-up and the timings).  Returns the per-repeat
+    milliseconds."""
     s = make_solver(nw, wl, precision, pipeline)
-    if warmup:
-        s.run(wl.dt, warmup)
-    reload(s, wl)
-    nw.lib().nw_dev_sync()
     t0 = time.perf_counter()
-    ms = s.time(wl.dt, steps)  # HIP events on the solver's stream around exactly `steps` steps
-    nw.lib().nw_dev_sync()
+    ms = s.timeRepeats(wl.dt, steps, repeats, warmup=warmup)
     wall = time.perf_counter() - t0
     info = s.info()
     segs = s.segments() if hasattr(s, "segments") else None
     s.finalize()
-    return ms, wall, info, segs
+    return [float(v) for v in ms], wall, info, segs
 
 
 def rel_l2(a, b, w):
@@ -87,14 +78,38 @@ def rel_l2(a, b, w):
     return math.sqrt(num / den)
 
 
-def cpu_baseline(dt, budget_s=12.0):
+def host_threads():
+    """The host cores this process may use, as `nproc` counts them: OMP_NUM_THREADS
+    when set, else the CPUs of the process's affinity mask."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.strip().isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
+
+
+def cpu_baseline(W, dt, budget_s=12.0):
     """The oracle (FP64 CPU restatement of the reference step) on the host
-    cores, on a bounded sample of the headline workload; plus one step on a
-    single thread."""
+    cores: a bounded sample of the headline workload on all usable cores and one
+    step of it on one thread; and the reference's CPU-runnable case in full (the
+    PEC cube 4^3, p = 4, TM110, CFL 0.5, 200 steps) with its error against the
+    analytic mode and its energy drift, on one thread and on all cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
 
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_threads()
     o = po.Mesh.box(8, (16, 16, 16), (0, 1, 0, 1, 0, 1), (1, 1, 1), jitter=0.1, seed=2)
     x, y, z = o.x, o.y, o.zc
     zero = np.zeros_like(x)
@@ -107,18 +122,29 @@ def cpu_baseline(dt, budget_s=12.0):
     t1 = s.run(dt, 1)  # one step sizes the sample
     n = max(1, min(1000, int(budget_s / max(t1, 1e-3))))
     t = s.run(dt, n)
-    cpu_model = ""
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                cpu_model = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
+    del s
+    # C1 in full: 200 steps
+    c1 = W.c1()
+    oc = po.Mesh.box(5, (4, 4, 4), (0, 1, 0, 1, 0, 1))
+    st1 = W.pack(W.tm_mode(oc.x, oc.y))
+    exact = W.pack(W.tm_mode(oc.x, oc.y, c1.steps * c1.dt))
+    w1 = oc.jac * np.tile(oc.w3, oc.nelt)
+    c1_out = {"steps": c1.steps, "dt": c1.dt, "nodes": oc.npts}
+    for label, nthr in (("1_thread", 1), ("all_cores", threads)):
+        po.set_threads(nthr)
+        sc = po.Solver(oc, 1.0)
+        sc.upload(st1)
+        e0 = oc.energy(st1)
+        tc = sc.run(c1.dt, c1.steps)
+        got = sc.download()
+        c1_out[label] = {"dof_updates_per_s": 6 * oc.npts * c1.steps / tc, "seconds": tc, "threads": nthr,
+                         "rel_l2_vs_analytic": rel_l2(got, exact, w1), "energy_drift": (oc.energy(got) - e0) / e0}
+    po.set_threads(threads)
     return {"value": 6 * o.npts * n / t, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
             "sample": "%d LSRK45 step(s) of the same 16^3 n1=8 jittered periodic mesh, FP64, %d OpenMP "
-                      "threads (%s), %.1f s" % (n, threads, cpu_model, t),
-            "value_1_thread": 6 * o.npts / t_one}
+                      "threads (%s), %.1f s" % (n, threads, cpu_model(), t),
+            "value_1_thread": 6 * o.npts / t_one, "cpu_model": cpu_model(), "os_cpu_count": os.cpu_count(),
+            "c1_full_run": c1_out}
 
 
 def load_traffic(name):
@@ -209,20 +235,59 @@ def case_loop(nw, wl, precision, steps=200, out_freq=10):
     return out
 
 
-def other_config(nw, W, name, steps):
+def run_state(nw, wl, precision, steps):
+    """The workload advanced `steps` steps; returns the live solver."""
+    s = make_solver(nw, wl, precision)
+    s.run(wl.dt, steps)
+    return s
+
+
+def other_config(nw, W, name, steps, error_steps=True):
+    """Rate of the fused FP32 path (and FP64 at the 64^3 share) on a workload,
+    and its stated error: the FP32 state after the workload's full step count
+    against the FP64 path's (relative L2, J w_i w_j w_k weighted, reduced on the
+    device); for the batched cavities also every instance's FP32 and FP64 error
+    against its closed-form TM_mn0 mode."""
     wl = W.BUILDERS[name]()
-    out = {"nodes": wl.npts, "elements": wl.nelt, "instances": len(wl.meshes), "description": wl.info["description"]}
+    out = {"nodes": wl.npts, "elements": wl.nelt, "instances": len(wl.meshes), "description": wl.info["description"],
+           "dt": wl.dt}
     precs = ["fp32", "fp64"] if name == "c5" else ["fp32"]
     for prec in precs:
-        ms, _, info, segs = time_path(nw, wl, prec, "fused", steps, 2)
+        reps, _, info, segs = time_path(nw, wl, prec, "fused", steps, 2, repeats=3)
+        ms = float(np.median(reps))
         s_u, s_k = SIZES[prec]
         b = algorithmic_bytes_per_step(wl.npts, wl.nelt, s_u, s_k)
         out[prec] = {"dof_updates_per_s": 6 * wl.npts * steps / (ms * 1e-3), "ms_per_step": ms / steps,
+                     "ms_per_step_repeats": [r / steps for r in reps],
                      "algorithmic_GBps": b / (ms / steps * 1e-3) / 1e9,
                      "hbm_frac": b / (ms / steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "kernels_per_step": info["kernels_per_step"]}
         if segs:
             out[prec]["segments"] = segs
+    if error_steps:
+        n = wl.steps
+        s64 = run_state(nw, wl, "fp64", n)
+        ref = [s64.downloadState() if len(wl.meshes) == 1 else s64.downloadState(x) for x in range(len(wl.meshes))]
+        s32 = run_state(nw, wl, "fp32", n)
+        errs = [s32.relL2(ref[x], x) for x in range(len(wl.meshes))]
+        err = {"steps": n, "fp32_vs_fp64_rel_l2": max(errs)}
+        if len(wl.meshes) > 1:
+            err["fp32_vs_fp64_rel_l2_median"] = float(np.median(errs))
+            an32, an64 = [], []
+            t = n * wl.dt
+            for x, (k, p, eps, mm, nn, ph) in enumerate(wl.info["instances"]):
+                xs, ys, _ = wl.meshes[x].coords()
+                exact = W.pack(W.tm_mode(xs, ys, t, mm, nn, eps, 1.0, ph))
+                an32.append(s32.relL2(exact, x))
+                an64.append(s64.relL2(exact, x))
+            err["fp32_vs_analytic_rel_l2_max"] = max(an32)
+            err["fp32_vs_analytic_rel_l2_median"] = float(np.median(an32))
+            err["fp64_vs_analytic_rel_l2_max"] = max(an64)
+            err["fp64_vs_analytic_rel_l2_median"] = float(np.median(an64))
+            err["fp32_minus_fp64_analytic_error_max"] = max(abs(a - b) for a, b in zip(an32, an64))
+        s32.finalize()
+        s64.finalize()
+        out["error"] = err
     del wl
     return out
 
@@ -234,6 +299,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32_k64", "fp64"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-errors", action="store_true",
+                    help="skip the full-length FP32-vs-FP64 runs of the other workloads")
     ap.add_argument("--accuracy-steps", type=int, default=1000)
     ap.add_argument("--configs", default="c3,c4,c5", help="other workloads to time (comma list, or 'none')")
     ap.add_argument("--profile-workload", default="c2", choices=["c2", "c3", "c5"],
@@ -260,7 +327,10 @@ def main():
     npts, nelt, dt = wl.npts, wl.nelt, wl.dt
     dof = 6 * npts
 
-    ms, wall, sinfo, segs = time_path(nw, wl, args.precision, "fused", args.steps, args.warmup)
+    # headline: W warm-up steps, then 5 back-to-back timings of exactly K steps;
+    # the value is their median
+    reps, wall, sinfo, segs = time_path(nw, wl, args.precision, "fused", args.steps, args.warmup, repeats=5)
+    ms = float(np.median(reps))
     value = dof * args.steps / (ms * 1e-3)
     s_u, s_k = SIZES[args.precision]
     b_step = algorithmic_bytes_per_step(npts, nelt, s_u, s_k)
@@ -271,19 +341,16 @@ def main():
 
     # comparison paths (fewer steps: they are slower)
     cmp_steps = max(10, args.steps // 5)
-    ms64, _, _, _ = time_path(nw, wl, "fp64", "fused", cmp_steps, 5)
-    ms64p, _, _, _ = time_path(nw, wl, "fp64", "plain", cmp_steps, 5)
+    ms64 = float(np.median(time_path(nw, wl, "fp64", "fused", cmp_steps, 5, repeats=3)[0]))
+    ms64p = float(np.median(time_path(nw, wl, "fp64", "plain", cmp_steps, 5, repeats=3)[0]))
     v64 = dof * cmp_steps / (ms64 * 1e-3)
     v64p = dof * cmp_steps / (ms64p * 1e-3)
     k64 = None
     if args.precision == "fp32":  # FP32 fields with FP64 LSRK registers (112 B per DOF-update)
-        msk, _, _, _ = time_path(nw, wl, "fp32_k64", "fused", cmp_steps, 5)
+        msk = float(np.median(time_path(nw, wl, "fp32_k64", "fused", cmp_steps, 5, repeats=3)[0]))
         b_k64 = algorithmic_bytes_per_step(npts, nelt, 4, 8)
         k64 = {"value": dof * cmp_steps / (msk * 1e-3), "ms_per_step": msk / cmp_steps,
                "hbm_frac": b_k64 * cmp_steps / (msk * 1e-3) / 1e9 / HBM_PEAK_GBS}
-
-    # repeats of the timed loop (median reported beside the contract's single timing)
-    reps = [time_path(nw, wl, args.precision, "fused", args.steps, 2)[0] / args.steps for _ in range(5)]
 
     # stated error: the measured precision vs the FP64 path after 1000 steps,
     # and the energy drift (U_end - U_0) / U_0 of both (upwind: dissipative)
@@ -306,9 +373,9 @@ def main():
     others = {}
     if args.configs and args.configs != "none":
         for name in args.configs.split(","):
-            others[name] = other_config(nw, W, name.strip(), 20)
+            others[name] = other_config(nw, W, name.strip(), 20, error_steps=not args.no_errors)
 
-    cpu = None if args.no_cpu else cpu_baseline(dt)
+    cpu = None if args.no_cpu else cpu_baseline(W, dt)
 
     line = {
         "metric": METRIC,
@@ -318,6 +385,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms / args.steps,
+        "timing": "median of 5 back-to-back timings of exactly `steps` steps after `warmup` untimed steps "
+                  "(HIP events on the solver's stream)",
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -342,8 +411,7 @@ def main():
         "rel_l2_vs_fp64": acc,
         "rel_l2_steps": args.accuracy_steps,
         "energy_drift": drift,
-        "ms_per_step_repeats": reps,
-        "ms_per_step_median": float(np.median(reps)),
+        "ms_per_step_repeats": [r / args.steps for r in reps],
         "tm110_c1": c1,
         "case_loop": loop,
         "other_configs": others,

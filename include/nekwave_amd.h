@@ -149,6 +149,10 @@ int nw_dg_step(nw_dg* s, double dt, double time);
 int nw_dg_run(nw_dg* s, double dt, int nsteps);
 /* nsteps steps timed with HIP events on the solver's stream (elapsed_ms). */
 int nw_dg_time(nw_dg* s, double dt, int nsteps, double* elapsed_ms);
+/* `warmup` untimed steps, then `repeats` timings of exactly nsteps steps each
+ * (elapsed_ms[0..repeats-1]), all queued back to back on the solver's stream
+ * with one host wait at the end. */
+int nw_dg_time_repeats(nw_dg* s, double dt, int warmup, int nsteps, int repeats, double* elapsed_ms);
 int nw_dg_sync(nw_dg* s);
 
 typedef struct nw_diag {
@@ -230,12 +234,16 @@ int nw_launch_compute_flux(const double* state, const double* fEN, const double*
                            double c0, int64_t totalFacePoints, int64_t stateStride, const int* volIdxMinus,
                            const double* eps, const double* mu, const int* ghost_elem, int np, int64_t npts,
                            void* stream);
-/* launch_add_flux (dg_kernels.hpp:46-50; dg_kernels.cu:316-338).  Deterministic:
- * each node gathers its face points in face order; requires the reference's
- * element-major face-point layout (dg_solver.cpp:76-112), so volIdxMinus is
- * implied and n1 is needed instead. */
-int nw_launch_add_flux(const double* flux, const double* dA, double* rhs, int64_t totalFacePoints, int64_t npts,
-                       int n1, void* stream);
+/* launch_add_flux (dg_kernels.hpp:46-50; dg_kernels.cu:316-338), the reference's
+ * arguments: rhs[c npts + volIdxMinus[p]] += dA[p] flux[...] (E rows from the
+ * flux's H-half order of dg_kernels.cu:329-336).  Where the face points follow
+ * the element-major layout of dg_solver.cpp:76-112 (sizes tfp = 6 n1^2 nelt,
+ * npts = n1^3 nelt and volIdxMinus as that layout puts them), each node gathers
+ * its points in face order 0..5: deterministic, no atomics.  Any point whose
+ * volIdxMinus departs from that layout is added with FP64 atomics as the
+ * reference adds every point. */
+int nw_launch_add_flux(const double* flux, const int* volIdxMinus, const double* dA, double* rhs,
+                       int64_t totalFacePoints, int64_t npts, void* stream);
 /* launch_inv_mass (dg_kernels.hpp:55-59; dg_kernels.cu:353-382); eps/mu per element or NULL. */
 int nw_launch_inv_mass(double* rhs, const double* jac, const double* w3, int nelt, int nxyz, int64_t npts,
                        const double* eps, const double* mu, void* stream);

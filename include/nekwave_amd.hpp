@@ -201,6 +201,7 @@ struct Config {
     double dt = -1.0;        // < 0: from cfl
     int numSteps = 100;
     int maxSteps = 100;
+    bool hasExplicitMaxSteps = false;  // a step count was given (config.hpp:29, 182-185)
     double finalTime = -1.0;
     int outputFreq = 10;
     double c0 = 1.0;         // 0 central, 1 upwind
@@ -224,6 +225,19 @@ struct Config {
     double packetSigma = 0.30;
     double packetX0 = std::nan("");
     std::string outputDir = "output";  // "none" keeps probe records in memory only
+    // the registry entry "wave_type" (Config::set, config.hpp:127-129): the built-in
+    // initial condition when Case::setWaveType was not called (case.cpp:163)
+    std::string waveType;
+
+    // Step limit as Case::preprocess settles it (case.cpp:429-436 after
+    // config.hpp:446-450): a non-default or explicit step count wins; a positive
+    // finalTime with no step count given runs to finalTime (-1: no limit).
+    int effectiveMaxSteps() const {
+        if (maxSteps != 100) return maxSteps;
+        if (numSteps != 100) return numSteps;
+        if (hasExplicitMaxSteps || finalTime <= 0.0) return 100;
+        return -1;
+    }
 };
 
 class Case;
@@ -242,7 +256,8 @@ class Case {
     // 247-372): "gaussian", "bloch"/"periodic", "multimode", "packet"; "3dboxper"
     // builds a [-1, 1]^3 box on [0, 2 pi]^3 instead (Mesh::rescale, box meshes).
     void setWaveType(const std::string& w) { waveType_ = w; }
-    const std::string& waveType() const { return waveType_; }
+    // setWaveType's value, else the config's (case.cpp:163)
+    const std::string& waveType() const { return waveType_.empty() ? config_.waveType : waveType_; }
     void addProbe(double x, double y, double z) { customProbes_.push_back({{x, y, z}}); }
     void addRelativeProbe(double rx, double ry, double rz) { customRel_.push_back({{rx, ry, rz}}); }
 
@@ -276,7 +291,7 @@ class Case {
         const bool boxGiven = config_.nelx > 0 && config_.nely > 0 && config_.nelz > 0;
         double b[6] = {config_.xmin, config_.xmax, config_.ymin, config_.ymax, config_.zmin, config_.zmax};
         if (!boxGiven) b[0] = b[2] = b[4] = -1.0, b[1] = b[3] = b[5] = 1.0;
-        if (waveType_ == "3dboxper" && std::abs(b[0] + 1.0) < 1e-3 && std::abs(b[1] - 1.0) < 1e-3)
+        if (waveType() == "3dboxper" && std::abs(b[0] + 1.0) < 1e-3 && std::abs(b[1] - 1.0) < 1e-3)
             b[0] = b[2] = b[4] = 0.0, b[1] = b[3] = b[5] = 2.0 * M_PI;  // case.cpp:165-176
         if (boxGiven)
             mesh_->createBoxMesh(config_.nelx, config_.nely, config_.nelz, b[0], b[1], b[2], b[3], b[4], b[5],
@@ -304,7 +319,7 @@ class Case {
             for (int k = 0; k < npts; ++k)
                 ic_(x[k], y[k], z[k], state_[k], state_[npts + k], state_[2 * size_t(npts) + k],
                     state_[3 * size_t(npts) + k], state_[4 * size_t(npts) + k], state_[5 * size_t(npts) + k]);
-        } else if (!waveType_.empty()) {
+        } else if (!waveType().empty()) {
             builtinIc();
         }
         currentTime_ = 0.0;
@@ -322,7 +337,7 @@ class Case {
         solver_.reset(new DgSolver(o));
         if (!solver_->initialize(*mesh_, config_.c0)) throw Error(NW_ERR_STATE, nw_last_error());
         solver_->uploadState(state_.data(), state_.size());
-        const int maxSteps = config_.maxSteps, outFreq = config_.outputFreq > 1 ? config_.outputFreq : 1;
+        const int maxSteps = config_.effectiveMaxSteps(), outFreq = config_.outputFreq > 1 ? config_.outputFreq : 1;
         const double finalTime = config_.finalTime;
         int step = 0;
         history_.clear();
@@ -428,7 +443,8 @@ class Case {
         const size_t n = x.size();
         double *Ez = &state_[2 * n], *Hx = &state_[3 * n], *Hy = &state_[4 * n];
         const double lx = c.xmax - c.xmin, ly = c.ymax - c.ymin;
-        if (waveType_ == "bloch" || waveType_ == "periodic") {
+        const std::string& w = waveType();
+        if (w == "bloch" || w == "periodic") {
             const double a = c.angleDeg, rad = a * M_PI / 180.0;
             double kx, ky;
             if (std::abs(a - 0.0) < 1e-4) kx = 2.0 * M_PI / lx, ky = 0.0;
@@ -463,13 +479,13 @@ class Case {
                 Hy[k] = -ca * e;
             }
             numModesUsed_ = nm;
-        } else if (waveType_ == "multimode") {
+        } else if (w == "multimode") {
             for (size_t k = 0; k < n; ++k) {
                 double e = 0.0;
                 for (int m = 1; m <= 8; ++m) e += (1.0 / m) * std::sin(m * M_PI / lx * (x[k] - c.xmin));
                 Ez[k] = e * std::sin(M_PI * (y[k] - c.ymin) / ly);
             }
-        } else if (waveType_ == "packet") {
+        } else if (w == "packet") {
             const double ck = std::isnan(c.carrierK) ? 4.0 * M_PI : c.carrierK;
             const double x0 = std::isnan(c.packetX0) ? c.xmin + 0.25 * lx : c.packetX0;
             for (size_t k = 0; k < n; ++k) {
@@ -477,7 +493,7 @@ class Case {
                 Ez[k] = std::exp(-(d * d) / (2.0 * c.packetSigma * c.packetSigma)) * std::sin(ck * (x[k] - c.xmin)) *
                         std::sin(M_PI * (y[k] - c.ymin) / ly);
             }
-        } else if (waveType_ == "gaussian") {
+        } else if (w == "gaussian") {
             for (size_t k = 0; k < n; ++k) Ez[k] = std::exp(-c.pulseSigma * (x[k] * x[k] + y[k] * y[k] + z[k] * z[k]));
         }
     }

@@ -286,6 +286,13 @@ class DgSolver:
         check(lib().nw_dg_time(self._handle(), float(dt), int(nsteps), C.byref(ms)))
         return ms.value
 
+    def timeRepeats(self, dt, nsteps, repeats, warmup=0):
+        """`warmup` untimed steps, then `repeats` back-to-back timings of exactly
+        `nsteps` steps (ms each, HIP events on the solver's stream)."""
+        ms = np.zeros(int(repeats))
+        check(lib().nw_dg_time_repeats(self._handle(), float(dt), int(warmup), int(nsteps), int(repeats), _dp(ms)))
+        return ms
+
     def diagnostics(self):
         o = _lib.Diag()
         check(lib().nw_dg_diag(self._handle(), C.byref(o)))

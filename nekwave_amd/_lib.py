@@ -75,6 +75,7 @@ SIGNATURES = {
     "nw_dg_step": (i, [vp, d, d]),
     "nw_dg_run": (i, [vp, d, i]),
     "nw_dg_time": (i, [vp, d, i, dp]),
+    "nw_dg_time_repeats": (i, [vp, d, i, i, i, dp]),
     "nw_dg_sync": (i, [vp]),
     "nw_dg_diag": (i, [vp, C.POINTER(Diag)]),
     "nw_dg_rel_l2": (i, [vp, i, dp, C.c_size_t, dp]),
@@ -94,7 +95,7 @@ SIGNATURES = {
     "nw_launch_volume_curl": (i, [vp] * 18 + [d, i, i, vp]),
     "nw_launch_restrict_faces": (i, [vp, vp, vp, vp, i64, i64, vp]),
     "nw_launch_compute_flux": (i, [vp] * 9 + [d, i64, i64, vp, vp, vp, vp, i, i64, vp]),
-    "nw_launch_add_flux": (i, [vp, vp, vp, i64, i64, i, vp]),
+    "nw_launch_add_flux": (i, [vp, vp, vp, vp, i64, i64, vp]),
     "nw_launch_inv_mass": (i, [vp, vp, vp, i, i, i64, vp, vp, vp]),
     "nw_launch_lsrk45_update": (i, [vp, vp, vp, d, d, d, i64, i64, vp]),
     "nw_launch_pack_halo": (i, [vp, vp, vp, vp, vp, i, i64, vp]),

@@ -69,7 +69,7 @@ def probes_json_text(order, num_modes, points, domain_lx, elements_x, mode_k=())
 
 class Config:
     _ALIASES = {"n": "order", "polynomial_order": "order", "num_steps": "num_steps", "nsteps": "num_steps",
-                "max_steps": "num_steps", "t_final": "final_time", "final_time": "final_time",
+                "max_steps": "num_steps", "steps": "num_steps", "t_final": "final_time", "final_time": "final_time",
                 "output_frequency": "output_frequency", "outputfreq": "output_frequency",
                 "flux": "flux_type", "mesh": "mesh_file", "file": "mesh_file", "path": "mesh_file",
                 "meshfile": "mesh_file", "probe": "probes", "probe_rel": "relative_probes",
@@ -80,6 +80,9 @@ class Config:
         self.cfl = -1.0
         self.dt = -1.0
         self.num_steps = 100
+        # whether num_steps / max_steps / steps was given (Config::hasExplicitMaxSteps,
+        # config.hpp:29, 182-185): without it a positive final_time alone ends the run
+        self.explicit_steps = False
         self.final_time = -1.0
         self.output_frequency = 10
         self.c0 = 1.0
@@ -104,6 +107,9 @@ class Config:
         self.packet_sigma = 0.30
         self.packet_x0 = None
         self.output_dir = "output"
+        # the registry entry "wave_type" (Config::set, config.hpp:127-129); Case uses it
+        # when setWaveType was not called (case.cpp:163)
+        self.wave_type = ""
         for k, v in kw.items():
             self.set(k, v)
 
@@ -124,8 +130,10 @@ class Config:
                 if len(v3) >= 3:
                     getattr(self, k).append(tuple(v3[:3]))
             return
-        if not hasattr(self, k):
+        if not hasattr(self, k) or k == "explicit_steps":
             raise KeyError("unknown config key %r" % key)
+        if k == "num_steps":
+            self.explicit_steps = True
         cur = getattr(self, k)
         if cur is None:
             val = float(val)
@@ -149,6 +157,10 @@ class Config:
             elif not k.startswith("_"):
                 flat[k] = v
         for k, v in flat.items():
+            if k.lower() == "wave_type":  # config.hpp:146-149: a file cannot choose the wave type
+                print("[CONFIG] Notice: 'wave_type' in configuration file is ignored (wave type must be "
+                      "specified directly in code).")
+                continue
             if isinstance(v, float):
                 v = _six_sig(v)
             try:
@@ -156,6 +168,37 @@ class Config:
             except KeyError:
                 pass
         return c
+
+    def max_steps(self):
+        """Step limit as Case::preprocess settles it (case.cpp:429-436 after
+        config.hpp:446-450): an explicit or non-default step count wins; a positive
+        final_time with no step count given runs to final_time (-1: no limit)."""
+        if self.num_steps != 100:
+            return self.num_steps
+        if self.explicit_steps or self.final_time <= 0.0:
+            return 100
+        return -1
+
+
+def step_schedule(dt, max_steps, final_time):
+    """The reference's time loop (case.cpp:562-583): yields (step, dt, time after
+    the step, whether it is the last step).  It stops at max_steps (when > 0) or
+    once time reaches final_time (when > 0), the last dt clamped to land on it."""
+    step, t = 0, 0.0
+    while True:
+        if max_steps > 0 and step >= max_steps:
+            return
+        if final_time > 0.0 and t >= final_time - 1e-13:
+            return
+        if max_steps <= 0 and final_time <= 0.0:
+            raise ValueError("no termination: neither a step limit nor a final time")
+        d = dt
+        if final_time > 0.0 and t + d > final_time:
+            d = final_time - t
+        step += 1
+        t += d
+        final = (max_steps > 0 and step >= max_steps) or (final_time > 0.0 and t >= final_time - 1e-13)
+        yield step, d, t, final
 
 
 class Case:
@@ -190,7 +233,8 @@ class Case:
         self.wave_type = str(wave_type)
 
     def waveType(self):
-        return self.wave_type
+        """The wave type in force: setWaveType's, else the config's (case.cpp:163)."""
+        return self.wave_type or self.config.wave_type
 
     def addProbe(self, x, y, z):
         self.custom_probes.append((float(x), float(y), float(z)))
@@ -254,7 +298,7 @@ class Case:
                 self.mesh.createBoxMesh(3, 3, 3, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0)
         else:  # the reference's default 3x3x3 box (case.cpp:158-161)
             self.mesh.createBoxMesh(3, 3, 3, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0)
-        if self.wave_type == "3dboxper":  # case.cpp:165-176
+        if self.waveType() == "3dboxper":  # case.cpp:165-176
             corners = self.mesh.getElementCorners()
             if abs(corners[..., 0].min() + 1.0) < 1e-3 and abs(corners[..., 0].max() - 1.0) < 1e-3:
                 lo, hi = corners.reshape(-1, 3).min(0), corners.reshape(-1, 3).max(0)
@@ -289,7 +333,7 @@ class Case:
         st = self.state
         lx, ly = c.xmax - c.xmin, c.ymax - c.ymin
         xt, yt = x - c.xmin, y - c.ymin
-        w = self.wave_type
+        w = self.waveType()
         if w in ("bloch", "periodic"):
             ang = c.angle_deg
             rad = ang * math.pi / 180.0
@@ -343,7 +387,7 @@ class Case:
         self.solver = DgSolver(precision=c.precision, pipeline=c.pipeline)
         self.solver.initialize(self.mesh, c.c0)
         self.solver.uploadState(self.state)
-        max_steps, final_time = c.num_steps, c.final_time
+        max_steps, final_time = c.max_steps(), c.final_time
         out_freq = max(1, c.output_frequency)
         self.probe_points = self._probe_list()
         nprobe = self.probe_points.shape[0]
@@ -366,7 +410,6 @@ class Case:
         # The reference's step schedule (case.cpp:562-583); the steps between two
         # outputs with equal dt go to the device as one multi-step launch with
         # one synchronisation instead of a synchronised step() each.
-        step = 0
         chunk = [0.0, 0]  # dt, steps not launched yet
 
         def flush():
@@ -374,27 +417,18 @@ class Case:
                 self.solver.run(chunk[0], chunk[1])
                 chunk[1] = 0
 
-        while True:
-            if max_steps > 0 and step >= max_steps:
-                break
-            if final_time > 0.0 and self.current_time >= final_time - 1e-13:
-                break
-            dt = self.dt
-            if final_time > 0.0 and self.current_time + dt > final_time:
-                dt = final_time - self.current_time
+        for step, dt, t, final in step_schedule(self.dt, max_steps, final_time):
             if chunk[1] and dt != chunk[0]:
                 flush()
             chunk[0] = dt
             chunk[1] += 1
-            step += 1
-            self.current_time += dt
+            self.current_time = t
             self.current_step = step
             if nprobe:
                 pending.append((step, self.current_time))
                 if len(pending) == cap:  # the device history is full: fetch it before the next step
                     flush()
                     self._fetch_probes(pending, out)
-            final = (max_steps > 0 and step >= max_steps) or (final_time > 0.0 and self.current_time >= final_time - 1e-13)
             if step % out_freq == 0 or final:
                 flush()
                 d = self.solver.diagnostics()

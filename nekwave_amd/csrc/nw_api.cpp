@@ -323,7 +323,11 @@ NW_API int nw_dg_run(nw_dg* s, double dt, int nsteps) {
 }
 NW_API int nw_dg_time(nw_dg* s, double dt, int nsteps, double* ms) {
     NW_CHECK_ARG(s && ms, "null argument");
-    return s->s.time(dt, nsteps, ms);
+    return s->s.time(dt, 0, nsteps, 1, ms);
+}
+NW_API int nw_dg_time_repeats(nw_dg* s, double dt, int warmup, int nsteps, int repeats, double* ms) {
+    NW_CHECK_ARG(s && ms, "null argument");
+    return s->s.time(dt, warmup, nsteps, repeats, ms);
 }
 NW_API int nw_dg_sync(nw_dg* s) {
     NW_CHECK_ARG(s, "null solver");
@@ -389,11 +393,11 @@ NW_API int nw_launch_compute_flux(const double* st, const double* fEN, const dou
                                npts, hs(stream)));
     return NW_OK;
 }
-NW_API int nw_launch_add_flux(const double* flux, const double* dA, double* rhs, int64_t tfp, int64_t npts, int n1,
-                              void* stream) {
-    NW_CHECK_ARG(n1 >= nw::kMinN1 && n1 <= nw::kMaxN1, "add_flux: bad n1");
-    NW_CHECK_ARG(tfp * n1 == 6 * npts, "add_flux: tfp must be 6 n1^2 nelt (element-major face layout)");
-    NW_TRY_HIP(nw::launch_lift(flux, dA, rhs, tfp, npts, n1, hs(stream)));
+NW_API int nw_launch_add_flux(const double* flux, const int* volIdxMinus, const double* dA, double* rhs, int64_t tfp,
+                              int64_t npts, void* stream) {
+    NW_CHECK_ARG(tfp >= 0 && npts >= 0, "add_flux: negative size");
+    NW_CHECK_ARG(tfp == 0 || (flux && volIdxMinus && dA && rhs), "add_flux: null pointer");
+    NW_TRY_HIP(nw::launch_add_flux(flux, volIdxMinus, dA, rhs, tfp, npts, hs(stream)));
     return NW_OK;
 }
 NW_API int nw_launch_inv_mass(double* rhs, const double* jac, const double* w3, int nelt, int nxyz, int64_t npts,

@@ -149,8 +149,11 @@ __global__ void k_flux(const double* __restrict__ st, const double* __restrict__
 
 // Deterministic lift: node (i,j,k) of element e gathers dA*flux from the face
 // points that map to it, faces in order 0..5 (the serial face-point order).
+// vM == nullptr: the element-major face layout is known to hold (the solver's
+// own tables).  Otherwise a face point counts here only if volIdxMinus names the
+// node the layout puts it at; k_lift_scatter adds the others.
 __global__ void k_lift(const double* __restrict__ flux, const double* __restrict__ dA, double* __restrict__ rhs,
-                       long long tfp, long long npts, int n1) {
+                       long long tfp, long long npts, int n1, const int* __restrict__ vM) {
     const int n2 = n1 * n1, np = n2 * n1;
     for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < npts; v += (long long)gridDim.x * blockDim.x) {
         const long long e = v / np;
@@ -169,12 +172,40 @@ __global__ void k_lift(const double* __restrict__ flux, const double* __restrict
         double r3 = rhs[3 * npts + v], r4 = rhs[4 * npts + v], r5 = rhs[5 * npts + v];
         for (int t = 0; t < nf; ++t) {
             const long long p = fb + pts[t];
+            if (vM && vM[p] != v) continue;
             const double a = dA[p];
             r0 += a * flux[3 * tfp + p]; r1 += a * flux[4 * tfp + p]; r2 += a * flux[5 * tfp + p];
             r3 += a * flux[p]; r4 += a * flux[tfp + p]; r5 += a * flux[2 * tfp + p];
         }
         rhs[v] = r0; rhs[npts + v] = r1; rhs[2 * npts + v] = r2;
         rhs[3 * npts + v] = r3; rhs[4 * npts + v] = r4; rhs[5 * npts + v] = r5;
+    }
+}
+
+// The face points k_lift did not take (n1 > 0: those whose volIdxMinus is not
+// the node the element-major layout puts them at; n1 == 0: all of them), added
+// as the reference adds every point: rhs[vM] += dA * flux with FP64 atomics
+// (dg_kernels.cu:316-338).  Points with vM outside [0, npts) are skipped.
+__global__ void k_lift_scatter(const double* __restrict__ flux, const double* __restrict__ dA, double* __restrict__ rhs,
+                               long long tfp, long long npts, int n1, const int* __restrict__ vM) {
+    const int n2 = n1 * n1, nf2 = 6 * n2;
+    for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < tfp; p += (long long)gridDim.x * blockDim.x) {
+        const long long v = vM[p];
+        if (n1 > 0) {
+            const long long e = p / nf2;
+            const int r = static_cast<int>(p - e * nf2), f = r / n2, pq = r - f * n2;
+            int i, j, k;
+            face_ijk(f, n1, pq % n1, pq / n1, i, j, k);
+            if (v == e * n2 * n1 + i + n1 * (j + n1 * k)) continue;
+        }
+        if (v < 0 || v >= npts) continue;
+        const double a = dA[p];
+        atomicAdd(&rhs[v], a * flux[3 * tfp + p]);
+        atomicAdd(&rhs[npts + v], a * flux[4 * tfp + p]);
+        atomicAdd(&rhs[2 * npts + v], a * flux[5 * tfp + p]);
+        atomicAdd(&rhs[3 * npts + v], a * flux[p]);
+        atomicAdd(&rhs[4 * npts + v], a * flux[tfp + p]);
+        atomicAdd(&rhs[5 * npts + v], a * flux[2 * tfp + p]);
     }
 }
 
@@ -549,7 +580,28 @@ hipError_t launch_flux(const double* st, const double* fEN, const double* fHN, c
 hipError_t launch_lift(const double* flux, const double* dA, double* rhs, long long tfp, long long npts, int n1,
                        hipStream_t s) {
     if (npts <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_lift, dim3(grid_for(npts)), dim3(kBlock), 0, s, flux, dA, rhs, tfp, npts, n1);
+    hipLaunchKernelGGL(k_lift, dim3(grid_for(npts)), dim3(kBlock), 0, s, flux, dA, rhs, tfp, npts, n1,
+                       static_cast<const int*>(nullptr));
+    return hipGetLastError();
+}
+
+hipError_t launch_add_flux(const double* flux, const int* vM, const double* dA, double* rhs, long long tfp,
+                           long long npts, hipStream_t s) {
+    if (tfp <= 0) return hipSuccess;
+    // n1 of the element-major layout, when the sizes admit one: tfp = 6 n1^2 nelt,
+    // npts = n1^3 nelt
+    int n1 = 0;
+    if (npts > 0 && (6 * npts) % tfp == 0) {
+        const long long c = 6 * npts / tfp;
+        if (c >= kMinN1 && c <= kMaxN1 && npts % (c * c * c) == 0 && tfp == 6 * c * c * (npts / (c * c * c)))
+            n1 = static_cast<int>(c);
+    }
+    if (n1 > 0) {
+        hipLaunchKernelGGL(k_lift, dim3(grid_for(npts)), dim3(kBlock), 0, s, flux, dA, rhs, tfp, npts, n1, vM);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_lift_scatter, dim3(grid_for(tfp)), dim3(kBlock), 0, s, flux, dA, rhs, tfp, npts, n1, vM);
     return hipGetLastError();
 }
 

@@ -26,6 +26,12 @@ hipError_t launch_flux(const double* st, const double* fEN, const double* fHN, c
                        long long npts, hipStream_t s);
 hipError_t launch_lift(const double* flux, const double* dA, double* rhs, long long tfp, long long npts, int n1,
                        hipStream_t s);
+// launch_add_flux with the reference's arguments (dg_kernels.hpp:46-50): nodes of
+// the element-major face layout gather their points in face order
+// (deterministic); points whose volIdxMinus departs from that layout are added
+// with atomics as the reference adds all of them.
+hipError_t launch_add_flux(const double* flux, const int* vM, const double* dA, double* rhs, long long tfp,
+                           long long npts, hipStream_t s);
 hipError_t launch_inv_mass(double* rhs, const double* jac, const double* w3, int nxyz, long long npts,
                            const double* eps, const double* mu, hipStream_t s);
 hipError_t launch_lsrk(double* st, double* k, const double* rhs, double a, double b, const double* dtp, double dt,

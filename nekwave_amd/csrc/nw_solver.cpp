@@ -83,6 +83,19 @@ static void append_group_order(const Mesh* m, int epw, int nelt, std::vector<int
     for (int g = 0; g < ng; ++g) out[base + size_t(g)] = static_cast<int>(key[size_t(g)] & 0xffffffffull);
 }
 
+unsigned long long mesh_signature(const Mesh& m) {
+    unsigned long long h = 1469598103934665603ull;
+    auto mix = [&](const void* p, size_t n) {
+        const unsigned char* b = static_cast<const unsigned char*>(p);
+        for (size_t x = 0; x < n; ++x) h = (h ^ b[x]) * 1099511628211ull;
+    };
+    mix(&m.n1, sizeof(m.n1));
+    mix(&m.nelt, sizeof(m.nelt));
+    mix(m.corners.data(), m.corners.size() * sizeof(double));
+    mix(m.fcode.data(), m.fcode.size() * sizeof(int));
+    return h;
+}
+
 Solver::~Solver() { release(); }
 
 void Solver::release() {
@@ -201,6 +214,7 @@ int Solver::init(const Mesh* const* meshes, int count, double c0_, const nw_opts
     }
     nelt = elems;
     npts = nodes;
+    mesh_sig = count == 1 ? mesh_signature(*meshes[0]) : 0;
     n1 = count == 1 ? meshes[0]->n1 : 0;
     np = count == 1 ? meshes[0]->np : 0;
     tfp = count == 1 ? meshes[0]->tfp() : 0;
@@ -545,7 +559,8 @@ int Solver::set_probes(const Mesh& m, int n, const double* xyz, int capacity) {
     NW_CHECK_ARG(n >= 0 && (n == 0 || xyz != nullptr), "bad probe list");
     NW_CHECK_ARG(capacity >= 1, "probe history capacity must be >= 1");
     NW_CHECK_ARG(inst.size() == 1, "probes need a single-mesh solver");
-    NW_CHECK_ARG(m.n1 == n1 && m.nelt == nelt, "probes: mesh does not match the solver's mesh");
+    NW_CHECK_ARG(m.n1 == n1 && m.nelt == nelt && mesh_signature(m) == mesh_sig,
+                 "probes: mesh is not the one the solver was created from (corners or faces differ)");
     NW_CHECK_ARG(n <= 65535, "too many probes (at most 65535)");
     NW_TRY_HIP(hipSetDevice(device));
     NW_TRY_HIP(hipStreamSynchronize(stream));
@@ -704,8 +719,9 @@ int Solver::step(double dt, int nsteps) {
     return NW_OK;
 }
 
-int Solver::time(double dt, int nsteps, double* ms) {
+int Solver::time(double dt, int warmup, int nsteps, int repeats, double* ms) {
     if (!initialized || !uploaded) { set_error("time before create/upload"); return NW_ERR_STATE; }
+    NW_CHECK_ARG(warmup >= 0 && nsteps >= 0 && repeats >= 1 && repeats <= 1024, "bad warmup/steps/repeats");
     NW_TRY_HIP(hipSetDevice(device));
     int st = set_dt(dt);
     if (st) return st;
@@ -715,13 +731,40 @@ int Solver::time(double dt, int nsteps, double* ms) {
         for (int p = 0; p < 2; ++p)
             if (!gexec[p] && (pipe == NW_PIPE_FUSED || p == 0) && (st = build_graph(p))) return st;
     }
-    NW_TRY_HIP(hipEventRecord(ev0, stream));
-    if ((st = launch_steps(dt, nsteps))) return st;
-    NW_TRY_HIP(hipEventRecord(ev1, stream));
-    NW_TRY_HIP(hipEventSynchronize(ev1));
-    float f = 0.0f;
-    NW_TRY_HIP(hipEventElapsedTime(&f, ev0, ev1));
-    *ms = f;
+    // warm-up steps, then `repeats` timed runs of exactly nsteps steps, all
+    // queued back to back: no host wait between them, one at the end
+    std::vector<hipEvent_t> ev(size_t(repeats) + 1, nullptr);
+    auto destroy = [&]() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    };
+    for (hipEvent_t& e : ev) {
+        const hipError_t he = hipEventCreate(&e);
+        if (he != hipSuccess) {
+            destroy();
+            set_error(std::string("hipEventCreate: ") + hipGetErrorString(he));
+            return NW_ERR_HIP;
+        }
+    }
+    hipError_t he = hipSuccess;
+    if ((st = launch_steps(dt, warmup))) { destroy(); return st; }
+    he = hipEventRecord(ev[0], stream);
+    for (int r = 0; r < repeats && he == hipSuccess && st == NW_OK; ++r) {
+        st = launch_steps(dt, nsteps);
+        if (st == NW_OK) he = hipEventRecord(ev[size_t(r) + 1], stream);
+    }
+    if (he == hipSuccess && st == NW_OK) he = hipEventSynchronize(ev[size_t(repeats)]);
+    for (int r = 0; r < repeats && he == hipSuccess && st == NW_OK; ++r) {
+        float f = 0.0f;
+        he = hipEventElapsedTime(&f, ev[size_t(r)], ev[size_t(r) + 1]);
+        ms[r] = f;
+    }
+    destroy();
+    if (st) return st;
+    if (he != hipSuccess) {
+        set_error(std::string("timing events: ") + hipGetErrorString(he));
+        return NW_ERR_HIP;
+    }
     return NW_OK;
 }
 

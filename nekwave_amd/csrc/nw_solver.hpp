@@ -22,6 +22,11 @@ struct Segment {
     long long goff[2] = {0, 0};  // offset of this segment's group order, per tile option
 };
 
+// FNV-1a hash of what makes a mesh this mesh: n1, corner coordinates (bit
+// patterns) and the face links.  Two meshes of the same shape but other bounds,
+// jitter or connectivity differ.
+unsigned long long mesh_signature(const Mesh& m);
+
 struct Solver {
     int device = -1;
     hipStream_t stream = nullptr;
@@ -29,6 +34,7 @@ struct Solver {
     int prec = 0, pipe = 1;
     bool use_graph = true, initialized = false, uploaded = false, media = false, autotune = false, tuned = false;
     int n1 = 0, np = 0, nelt = 0, kernels_per_step = 0;
+    unsigned long long mesh_sig = 0;  // mesh_signature of a single-mesh solver's mesh
     long long npts = 0, tfp = 0, nhalo = 0, stride = 0, device_bytes = 0;
     double c0 = 1.0, last_dt = -1.0;
     int cur = 0;
@@ -78,7 +84,7 @@ struct Solver {
     int download(int instance, double* host, size_t n);
     int step(double dt, int nsteps);
     int launch_steps(double dt, int nsteps);
-    int time(double dt, int nsteps, double* ms);
+    int time(double dt, int warmup, int nsteps, int repeats, double* ms);
     int diag(int instance, nw_diag* out);
     int rel_l2(int instance, const double* ref, size_t n, double* out);
     int set_probes(const Mesh& m, int n, const double* xyz, int capacity);

@@ -6,7 +6,8 @@
 //                         lifecycle with IC and post-processing hooks; the
 //                         .rea mesh restated as a 3x3x3 box on [-1,1]^3)
 //
-// Exit status 0 when every check passes.  Usage: reference_kats [fp64|fp32]
+// Exit status 0 when every check passes.  Usage: reference_kats [fp64|fp32|host]
+// ("host": only the checks that need no device -- built-in fields, step limit).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -195,9 +196,85 @@ static void BuiltinBlochIc() {
     std::printf("  PASSED\n");
 }
 
+// Built-in fields by value (case.cpp:330-372), host only: -ln(Ez) / r^2 is the
+// Gaussian's sigma at every node off the origin; the multimode sum at x = Lx/2,
+// y = Ly/2 is 1 - 1/3 + 1/5 - 1/7.  The wave type may come from the config
+// (Config::set "wave_type", case.cpp:163).
+static void BuiltinFieldValues() {
+    std::printf("[RUN] built-in gaussian and multimode values\n");
+    Config cfg;
+    cfg.order = 5;
+    cfg.nelx = 4;
+    cfg.nely = cfg.nelz = 2;
+    cfg.pulseSigma = 3.0;
+    cfg.waveType = "gaussian";
+    Case g;
+    g.setConfig(cfg);
+    g.preprocess();
+    EXPECT_TRUE(g.waveType() == "gaussian");
+    {
+        const auto& st = g.state();
+        const size_t n = st.size() / 6;
+        const auto x = g.mesh().getCoordX(), y = g.mesh().getCoordY(), z = g.mesh().getCoordZ();
+        int checked = 0;
+        for (size_t k = 0; k < n; ++k) {
+            const double r2 = x[k] * x[k] + y[k] * y[k] + z[k] * z[k];
+            if (r2 < 0.05) continue;
+            EXPECT_TRUE(std::abs(-std::log(st[2 * n + k]) / r2 - 3.0) < 1e-12);
+            ++checked;
+        }
+        EXPECT_TRUE(checked > 100);
+    }
+    cfg.waveType.clear();
+    cfg.xmin = 0.0, cfg.xmax = 2.0, cfg.ymin = 0.0, cfg.ymax = 1.0, cfg.zmin = 0.0, cfg.zmax = 1.0;
+    Case mm;
+    mm.setConfig(cfg);
+    mm.setWaveType("multimode");
+    mm.preprocess();
+    const auto& st = mm.state();
+    const size_t n = st.size() / 6;
+    const auto x = mm.mesh().getCoordX(), y = mm.mesh().getCoordY();
+    int at = 0;
+    for (size_t k = 0; k < n; ++k)
+        if (std::abs(x[k] - 1.0) < 1e-12 && std::abs(y[k] - 0.5) < 1e-12) {
+            EXPECT_TRUE(std::abs(st[2 * n + k] - (1.0 - 1.0 / 3 + 1.0 / 5 - 1.0 / 7)) < 1e-14);
+            ++at;
+        }
+    EXPECT_TRUE(at > 0);
+    std::printf("  PASSED\n");
+}
+
+// Step limit of Case::preprocess (case.cpp:429-436, config.hpp:446-450).
+static void StepLimit() {
+    std::printf("[RUN] step limit from num_steps / final_time\n");
+    Config c;
+    EXPECT_TRUE(c.effectiveMaxSteps() == 100);
+    c.finalTime = 1.0;
+    EXPECT_TRUE(c.effectiveMaxSteps() == -1);  // runs to finalTime
+    c.hasExplicitMaxSteps = true;
+    EXPECT_TRUE(c.effectiveMaxSteps() == 100);
+    c.hasExplicitMaxSteps = false;
+    c.maxSteps = 7;
+    EXPECT_TRUE(c.effectiveMaxSteps() == 7);
+    std::printf("  PASSED\n");
+}
+
 int main(int argc, char** argv) {
+    if (argc > 1 && std::strcmp(argv[1], "host") == 0) {  // host-side checks only (no device)
+        try {
+            BuiltinFieldValues();
+            StepLimit();
+        } catch (const Error& e) {
+            std::fprintf(stderr, "error %d: %s\n", e.status(), e.what());
+            return 2;
+        }
+        std::printf("%s\n", g_failed ? "FAILED" : "ALL PASSED");
+        return g_failed ? 1 : 0;
+    }
     if (argc > 1 && std::strcmp(argv[1], "fp32") == 0) g_prec = NW_PREC_FP32;
     try {
+        BuiltinFieldValues();
+        StepLimit();
         MaxwellStabilityTest();
         BuiltinBlochIc();
         BoxPecCase();

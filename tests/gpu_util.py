@@ -63,3 +63,12 @@ def random_state(o, seed=0):
     x, y, z = o.x, o.y, o.zc
     f = [np.sin(2 * math.pi * (x + rng.uniform())) * np.cos(math.pi * y) for _ in range(6)]
     return an.pack(f) + 0.1 * rng.standard_normal(6 * o.npts)
+
+
+def lift_numpy(vM, dA, fl, rhs0, T, npts):
+    """rhs[c npts + vM[p]] += dA[p] F_c[p] with the E rows taken from the flux's
+    second half (dg_kernels.cu:316-338), accumulated point by point with np.add.at."""
+    out = rhs0.copy()
+    for c, src in enumerate((3, 4, 5, 0, 1, 2)):
+        np.add.at(out, c * npts + vM, dA * fl[src * T:(src + 1) * T])
+    return out

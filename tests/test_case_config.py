@@ -1,0 +1,73 @@
+"""Case configuration rules that decide how long a run is and which built-in
+field it starts from (host only, no device):
+
+  * the step limit of Case::preprocess (case.cpp:429-436 after
+    config.hpp:446-450): a config with only final_time runs to final_time,
+    ceil(final_time / dt) steps with the last dt clamped (case.cpp:562-583);
+    an explicit step count wins;
+  * the wave type: setWaveType's value, else the registry's "wave_type" set in
+    code (Config::set, case.cpp:163); a configuration file cannot choose it
+    (config.hpp:146-149 ignores the key with a notice).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from nekwave_amd.case import Case, Config, step_schedule
+
+
+def test_final_time_alone_runs_to_final_time():
+    c = Config(final_time=1.0)
+    assert c.max_steps() == -1
+    dt = 0.3
+    sched = list(step_schedule(dt, c.max_steps(), c.final_time))
+    assert len(sched) == math.ceil(1.0 / dt) == 4
+    assert [s[0] for s in sched] == [1, 2, 3, 4]
+    assert sched[-1][1] == pytest.approx(0.1, abs=1e-15) and sched[-1][2] == pytest.approx(1.0, abs=1e-15)
+    assert [s[3] for s in sched] == [False, False, False, True]
+    c2 = Config.from_json({"time": {"final_time": 0.5}})
+    assert c2.max_steps() == -1 and len(list(step_schedule(0.1, c2.max_steps(), 0.5))) == 5
+
+
+def test_explicit_step_count_wins():
+    assert Config(final_time=1.0, num_steps=100).max_steps() == 100  # explicit, even at the default value
+    assert Config(final_time=1.0, max_steps=3).max_steps() == 3
+    assert Config.from_json({"steps": 7, "final_time": 9.0}).max_steps() == 7
+    assert Config().max_steps() == 100  # neither given: the default 100
+    sched = list(step_schedule(0.3, 3, 10.0))
+    assert len(sched) == 3 and sched[-1][3]
+    sched = list(step_schedule(0.3, 10, 0.5))  # final time first: clamped second step
+    assert len(sched) == 2 and sched[-1][1] == pytest.approx(0.2, abs=1e-15)
+
+
+def test_wave_type_from_config_in_code_but_not_from_a_file(capsys):
+    base = dict(order=4, nelx=2, nely=2, nelz=2)
+    c = Case(Config(wave_type="gaussian", pulse_sigma=3.0, **base))
+    c.preprocess()
+    assert c.waveType() == "gaussian"
+    x, y, z = c.mesh.coords()
+    np.testing.assert_allclose(c.state.reshape(6, -1)[2], np.exp(-3.0 * (x * x + y * y + z * z)), rtol=1e-15)
+    c2 = Case(Config(wave_type="gaussian", **base))
+    c2.setWaveType("multimode")  # setWaveType takes precedence
+    assert c2.waveType() == "multimode"
+    f = Config.from_json(dict(wave_type="gaussian", mesh=dict(nelx=2, nely=2, nelz=2, order=4)))
+    assert "ignored" in capsys.readouterr().out
+    assert f.wave_type == ""
+    c3 = Case(f)
+    c3.preprocess()
+    assert np.all(c3.state == 0.0)
+
+
+def test_cpp_case_host_checks():
+    """The C++ Case (include/nekwave_amd.hpp) on the host: built-in field values,
+    wave type from the config and the step limit (tests/cpp/reference_kats.cpp,
+    "host" mode: no device)."""
+    import os
+    import subprocess
+
+    from nekwave_amd import build as b
+
+    assert os.path.exists(b.KATS_BIN), "built by __graft_entry__.build()"
+    r = subprocess.run([b.KATS_BIN, "host"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "ALL PASSED" in r.stdout, r.stdout + r.stderr

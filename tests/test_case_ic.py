@@ -86,3 +86,53 @@ def test_hook_takes_precedence_over_the_wave_type():
     c.preprocess()
     u = c.state.reshape(6, -1)
     assert np.all(u[0] == 1.0) and np.all(u[2] == 0.0)
+
+
+# Values, not only properties: each built-in field is recovered from the node
+# values by a characterisation a plausible slip in its parameters fails.
+
+def test_gaussian_log_is_quadratic_in_r_with_slope_sigma():
+    """exp(-sigma r^2) (case.cpp:365-372): -ln(Ez) / r^2 equals sigma at every node
+    off the origin; exp(-r^2 / sigma), exp(-sigma r) or a shifted centre give
+    another constant or none."""
+    sigma = 3.0
+    c, x, y, z, u = _case("gaussian", xmin=-1.0, xmax=1.0, ymin=-1.0, ymax=1.0, zmin=-1.0, zmax=1.0,
+                          pulse_sigma=sigma)
+    r2 = x * x + y * y + z * z
+    sel = r2 > 0.05
+    np.testing.assert_allclose(-np.log(u[2][sel]) / r2[sel], sigma, rtol=1e-13)
+    c20, *_ , u20 = _case("gaussian", xmin=-1.0, xmax=1.0, ymin=-1.0, ymax=1.0, zmin=-1.0, zmax=1.0)
+    np.testing.assert_allclose(-np.log(u20[2][sel]) / r2[sel], 20.0, rtol=1e-13)  # config.hpp:34 default
+
+
+def test_multimode_partial_fourier_sums():
+    """sum_{m=1..8} (1/m) sin(m pi x / Lx) sin(pi y / Ly) (case.cpp:330-345): at
+    x = Lx/2 the sine factors are 1, 0, -1, 0, ... so Ez = 1 - 1/3 + 1/5 - 1/7; at
+    x = Lx/4 the factors are sqrt(2)/2 (1, sqrt 2, 1, 0, -1, -sqrt 2, -1, 0).  A
+    1/m^2 weight, a missing term or a y factor other than sin(pi y / Ly) fails."""
+    c, x, y, z, u = _case("multimode")  # Lx = 2, Ly = 1
+    mid_y = np.isclose(y, 0.5)
+    at_half = np.isclose(x, 1.0) & mid_y
+    at_quarter = np.isclose(x, 0.5) & mid_y
+    assert at_half.any() and at_quarter.any()
+    np.testing.assert_allclose(u[2][at_half], 1 - 1 / 3 + 1 / 5 - 1 / 7, rtol=1e-14)
+    h = math.sqrt(2.0) / 2.0
+    want = h * (1 + 1 / 3 - 1 / 5 - 1 / 7) + (1 / 2 - 1 / 6)
+    np.testing.assert_allclose(u[2][at_quarter], want, rtol=1e-14)
+    at_quarter_y = np.isclose(x, 1.0) & np.isclose(y, 0.25)
+    np.testing.assert_allclose(u[2][at_quarter_y], (1 - 1 / 3 + 1 / 5 - 1 / 7) * h, rtol=1e-14)
+
+
+def test_packet_envelope_vertex_and_width():
+    """env(x) sin(k (x - xmin)) sin(pi y / Ly), env = exp(-(x - x0)^2 / (2 s^2)),
+    k = 4 pi, x0 = xmin + Lx/4, s = 0.3 (case.cpp:346-364): dividing out the two
+    sines leaves ln env, a parabola whose vertex is x0 and whose curvature is
+    -1/(2 s^2)."""
+    c, x, y, z, u = _case("packet")
+    carrier = np.sin(4.0 * math.pi * x) * np.sin(math.pi * y)
+    sel = np.abs(carrier) > 0.2
+    ln_env = np.log(u[2][sel] / carrier[sel])
+    a2, a1, a0 = np.polyfit(x[sel], ln_env, 2)
+    assert a2 == pytest.approx(-1.0 / (2 * 0.3 * 0.3), rel=1e-9)
+    assert -a1 / (2 * a2) == pytest.approx(0.5, rel=1e-9)  # x0 = 0 + 2/4
+    assert a0 - a1 * a1 / (4 * a2) == pytest.approx(0.0, abs=1e-9)  # unit peak

@@ -10,6 +10,7 @@ import nekwave_amd as nw
 import pyoracle as po
 from nekwave_amd import DeviceBuffer as DB
 from nekwave_amd._lib import check, lib
+from gpu_util import lift_numpy
 
 pytestmark = pytest.mark.gpu
 
@@ -81,9 +82,38 @@ def test_restrict_flux_lift_parity(media, halo):
     np.testing.assert_allclose(got, fl, rtol=0, atol=1e-14 * np.abs(fl).max())
     d_rhs = DB.of(rhs0)
     d_fl.upload(fl)  # lift the oracle flux so the lift is checked on its own
-    check(lib().nw_launch_add_flux(d_fl.ptr, d_dA.ptr, d_rhs.ptr, T, o.npts, o.n1, None))
+    # the reference's argument list (dg_kernels.hpp:46-50)
+    check(lib().nw_launch_add_flux(d_fl.ptr, d_vM.ptr, d_dA.ptr, d_rhs.ptr, T, o.npts, None))
     check(lib().nw_dev_sync())
     np.testing.assert_allclose(d_rhs.download(np.float64, 6 * o.npts), rhs, rtol=0, atol=1e-14 * np.abs(rhs).max())
+
+
+def test_add_flux_with_a_foreign_face_layout():
+    """volIdxMinus that departs from the element-major layout (permuted points,
+    points sent to other nodes) and sizes that admit no n1: the points the layout
+    does not explain are added as the reference adds them."""
+    o = _mesh(False, False)
+    T, P = o.tfp, o.npts
+    rng = np.random.default_rng(21)
+    fl = rng.standard_normal(6 * T)
+    dA = rng.uniform(0.5, 1.5, T)
+    rhs0 = rng.standard_normal(6 * P)
+    vM = o.vM.copy()
+    sel = rng.choice(T, T // 5, replace=False)
+    vM[sel] = rng.integers(0, P, sel.size).astype(vM.dtype)
+    want = lift_numpy(vM, dA, fl, rhs0, T, P)
+    d_fl, d_dA, d_vM, d_rhs = DB.of(fl), DB.of(dA), DB.of(vM), DB.of(rhs0)
+    check(lib().nw_launch_add_flux(d_fl.ptr, d_vM.ptr, d_dA.ptr, d_rhs.ptr, T, P, None))
+    check(lib().nw_dev_sync())
+    np.testing.assert_allclose(d_rhs.download(np.float64, 6 * P), want, rtol=0, atol=1e-13 * np.abs(want).max())
+    # a subset of the face points (no n1 fits the sizes): every point scattered
+    T2 = T - 7
+    want2 = lift_numpy(vM[:T2], dA[:T2], np.concatenate([fl[c * T:c * T + T2] for c in range(6)]), rhs0, T2, P)
+    d_fl2 = DB.of(np.concatenate([fl[c * T:c * T + T2] for c in range(6)]))
+    d_rhs2 = DB.of(rhs0)
+    check(lib().nw_launch_add_flux(d_fl2.ptr, d_vM.ptr, d_dA.ptr, d_rhs2.ptr, T2, P, None))
+    check(lib().nw_dev_sync())
+    np.testing.assert_allclose(d_rhs2.download(np.float64, 6 * P), want2, rtol=0, atol=1e-13 * np.abs(want2).max())
 
 
 @pytest.mark.parametrize("media", [False, True])

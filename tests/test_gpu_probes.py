@@ -207,6 +207,11 @@ def test_probe_entry_points_reject_bad_calls():
     s.uploadState(random_state(o, 1))
     assert nw.lib().nw_dg_probe_sample(s._handle()) == _lib.NW_ERR_STATE
     assert nw.lib().nw_dg_set_probes(s._handle(), m2.handle, 1, ptr, 4) == _lib.NW_ERR_ARG
+    # same n1 and element count, other bounds: not the solver's mesh either
+    m3 = nw.Mesh(4)
+    m3.createBoxMesh(2, 2, 2, 0.0, 2.0, 0.0, 1.0, 0.0, 1.0)
+    assert nw.lib().nw_dg_set_probes(s._handle(), m3.handle, 1, ptr, 4) == _lib.NW_ERR_ARG
+    assert "not the one the solver was created from" in nw.lib().nw_last_error().decode()
     assert nw.lib().nw_dg_set_probes(s._handle(), m.handle, 1, ptr, 0) == _lib.NW_ERR_ARG
     s.setProbes(m, xyz, capacity=2)  # and the solver still works afterwards
     s.sampleProbes()

@@ -357,3 +357,18 @@ def test_self_halo_bitwise_equals_periodic():
     a = _run(m0, st, dt, 3)
     b = _run(m1, st, dt, 3)
     assert np.array_equal(a, b)
+
+
+def test_numpy_lift_equals_oracle_lift():
+    """The point-by-point lift the kernel-level tests use as their reference for
+    foreign face layouts equals the oracle's or_add_flux on the element-major
+    layout (the E rows come from the flux's H-ordered second half)."""
+    from gpu_util import lift_numpy
+
+    o = po.Mesh.box(4, (2, 2, 1), UNIT, (1, 0, 0), jitter=0.1, seed=3)
+    rng = np.random.default_rng(4)
+    fl = rng.standard_normal(6 * o.tfp)
+    rhs0 = rng.standard_normal(6 * o.npts)
+    want = po.add_flux(o, fl, rhs0)
+    got = lift_numpy(o.vM, o.dA, fl, rhs0, o.tfp, o.npts)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-14 * np.abs(want).max())
