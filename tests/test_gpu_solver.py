@@ -131,8 +131,6 @@ def test_segment_option_reports_matrix_volume():
 def test_device_rel_l2_matches_host(name, pipeline, precision):
     """nw_dg_rel_l2 (3dboxpec_test.cpp:76-119 norm, reduced on the device) equals
     the host formula on the downloaded state, and is 0 against the state itself."""
-    if pipeline == "plain" and name == "halo_media_p4":
-        pytest.skip("covered by the fused pipeline")
     m, o, c0 = _build(name)
     st0 = random_state(o, 7)
     dt = 0.5 * o.auto_dt()
