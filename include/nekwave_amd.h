@@ -256,6 +256,18 @@ int nw_launch_pack_halo(const double* state, double* sendBuf, const int* sendVol
 int nw_launch_unpack_halo(const double* recvBuf, double* state, const int* recvBufOffsets, const int* exchangeSizes,
                           int numHaloPoints, int64_t npts, int64_t stateStride, void* stream);
 
+/* Case::computeTotalEnergy, Case::getMaxE, Case::getMaxH (case.cpp:733-794) on a
+ * device state instead of the downloaded host copy: `state` in the reference
+ * layout (component c of node p at c stateStride + p, stateStride >= npts =
+ * nelt n1^3), jac per node (getJac), w the n1 1-D GLL weights (getGllW),
+ * eps/mu per element or NULL (NULL, NULL is the reference's energy).  Writes
+ * out[0] = 1/2 sum (eps |E|^2 + mu |H|^2) J w_i w_j w_k, out[1] = max |E|,
+ * out[2] = max |H| to the device array `out` (3 doubles), asynchronously on
+ * `stream`.  Partial sums go to a buffer the library allocates once per device
+ * (on that device's first call).  FP64, deterministic for a given npts. */
+int nw_launch_diagnostics(const double* state, int64_t stateStride, const double* jac, const double* w,
+                          const double* eps, const double* mu, int n1, int nelt, double* out, void* stream);
+
 /* Seeded synthetic inputs: std::mt19937_64(seed) with uniform = (x >> 11) 2^-53,
  * normal = Box-Muller first variate of (1 - uniform, uniform); n draws. */
 int nw_rand_uniform(uint64_t seed, size_t n, double* out);

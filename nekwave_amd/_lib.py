@@ -96,6 +96,7 @@ SIGNATURES = {
     "nw_launch_restrict_faces": (i, [vp, vp, vp, vp, i64, i64, vp]),
     "nw_launch_compute_flux": (i, [vp] * 9 + [d, i64, i64, vp, vp, vp, vp, i, i64, vp]),
     "nw_launch_add_flux": (i, [vp, vp, vp, vp, i64, i64, vp]),
+    "nw_launch_diagnostics": (i, [vp, i64, vp, vp, vp, vp, i, i, vp, vp]),
     "nw_launch_inv_mass": (i, [vp, vp, vp, i, i, i64, vp, vp, vp]),
     "nw_launch_lsrk45_update": (i, [vp, vp, vp, d, d, d, i64, i64, vp]),
     "nw_launch_pack_halo": (i, [vp, vp, vp, vp, vp, i, i64, vp]),

@@ -1,6 +1,7 @@
 // nw_api.cpp -- extern "C" boundary (include/nekwave_amd.h).
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <random>
 #include <string>
@@ -421,6 +422,47 @@ NW_API int nw_launch_unpack_halo(const double* buf, double* st, const int* off, 
                                  int64_t npts, int64_t S, void* stream) {
     NW_TRY_HIP(nw::launch_unpack<double>(buf, st, off, sizes, nh, npts, S, 1, hs(stream)));
     return NW_OK;
+}
+
+// Partial-sum buffer of nw_launch_diagnostics, one per device, allocated on the
+// device's first call and kept until the process ends.
+static std::mutex g_part_mu;
+static double* g_part[64] = {};
+
+static int check_diag_args(const double* st, int64_t S, const double* jac, const double* w, int n1, int nelt,
+                           const double* out) {
+    NW_CHECK_ARG(n1 >= nw::kMinN1 && n1 <= nw::kMaxN1, "diagnostics: n1 must be in [2, 10]");
+    NW_CHECK_ARG(nelt > 0, "diagnostics: nelt must be positive");
+    NW_CHECK_ARG(st && jac && w && out, "diagnostics: null pointer");
+    NW_CHECK_ARG(S >= (int64_t)nelt * n1 * n1 * n1, "diagnostics: stateStride < npts");
+    return NW_OK;
+}
+
+NW_API int nw_launch_diagnostics(const double* st, int64_t S, const double* jac, const double* w, const double* eps,
+                                 const double* mu, int n1, int nelt, double* out, void* stream) {
+    NW_GUARD_BEGIN
+    int rc = check_diag_args(st, S, jac, w, n1, nelt, out);
+    if (rc) return rc;
+    int dev = 0;
+    NW_TRY_HIP(hipGetDevice(&dev));
+    NW_CHECK_ARG(dev >= 0 && dev < 64, "diagnostics: device ordinal beyond 63");
+    double* part = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_part_mu);
+        if (!g_part[dev]) {
+            void* p = nullptr;
+            if (hipMalloc(&p, sizeof(double) * 3 * nw::kEnergyMaxBlocks) != hipSuccess) {
+                set_error("diagnostics: hipMalloc of the partial sums failed");
+                return NW_ERR_ALLOC;
+            }
+            g_part[dev] = static_cast<double*>(p);
+        }
+        part = g_part[dev];
+    }
+    const long long npts = (long long)nelt * n1 * n1 * n1;
+    NW_TRY_HIP(nw::launch_energy(st, S, npts, n1, jac, w, eps, mu, part, out, hs(stream)));
+    return NW_OK;
+    NW_GUARD_END
 }
 
 // ---------------------------------------------------------------- rng ----

@@ -421,6 +421,48 @@ __global__ void k_diag_final(const double* __restrict__ part, int n, double* __r
     }
 }
 
+// Case::computeTotalEnergy, getMaxE, getMaxH (case.cpp:733-794) on a device
+// state in the reference layout (component c of node p at c S + p): per block
+// the sum of 1/2 (eps |E|^2 + mu |H|^2) J w_i w_j w_k (the reference's
+// J[p] * w[i] * w[j] * w[k]) and the largest |E|^2, |H|^2; eps/mu per element
+// or NULL (the reference's 1/2 (|E|^2 + |H|^2) dV).
+__global__ void k_energy_partial(const double* __restrict__ st, long long S, long long npts, int n1,
+                                 const double* __restrict__ jac, const double* __restrict__ w,
+                                 const double* __restrict__ eps, const double* __restrict__ mu,
+                                 double* __restrict__ part) {
+    __shared__ double sh[3][kBlock];
+    const int n2 = n1 * n1, np = n2 * n1;
+    double en = 0.0, me = 0.0, mh = 0.0;
+    for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < npts; p += (long long)gridDim.x * blockDim.x) {
+        const long long e = p / np;
+        const int m = static_cast<int>(p - e * np);
+        const int i = m % n1, j = (m / n1) % n1, k = m / n2;
+        const double dV = jac[p] * w[i] * w[j] * w[k];
+        const double ex = st[p], ey = st[S + p], ez = st[2 * S + p];
+        const double hx = st[3 * S + p], hy = st[4 * S + p], hz = st[5 * S + p];
+        const double e2 = ex * ex + ey * ey + ez * ez, h2 = hx * hx + hy * hy + hz * hz;
+        if (eps || mu)
+            en += 0.5 * ((eps ? eps[e] : 1.0) * e2 + (mu ? mu[e] : 1.0) * h2) * dV;
+        else
+            en += 0.5 * (e2 + h2) * dV;
+        me = fmax(me, e2);
+        mh = fmax(mh, h2);
+    }
+    sh[0][threadIdx.x] = en; sh[1][threadIdx.x] = me; sh[2][threadIdx.x] = mh;
+    __syncthreads();
+    for (int r = blockDim.x / 2; r > 0; r >>= 1) {
+        if (threadIdx.x < r) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + r];
+            sh[1][threadIdx.x] = fmax(sh[1][threadIdx.x], sh[1][threadIdx.x + r]);
+            sh[2][threadIdx.x] = fmax(sh[2][threadIdx.x], sh[2][threadIdx.x + r]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x] = sh[0][0]; part[3 * blockIdx.x + 1] = sh[1][0]; part[3 * blockIdx.x + 2] = sh[2][0];
+    }
+}
+
 // Relative L2 distance to a reference state (3dboxpec_test.cpp:76-119):
 // sum over components and nodes of (u - ref)^2 dV and ref^2 dV, in FP64.
 // ref is in the reference's host layout (component blocks of npts).
@@ -670,6 +712,19 @@ hipError_t launch_diag(const R* u, long long cs, long long ns, long long npts, i
     hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s, part, nb, out);
     return hipGetLastError();
 }
+int energy_blocks(long long npts) {
+    const long long b = (npts + kBlock - 1) / kBlock;
+    return static_cast<int>(b < 1 ? 1 : (b > kEnergyMaxBlocks ? kEnergyMaxBlocks : b));
+}
+
+hipError_t launch_energy(const double* st, long long S, long long npts, int n1, const double* jac, const double* w,
+                         const double* eps, const double* mu, double* part, double* out, hipStream_t s) {
+    const int nb = energy_blocks(npts);
+    hipLaunchKernelGGL(k_energy_partial, dim3(nb), dim3(kBlock), 0, s, st, S, npts, n1, jac, w, eps, mu, part);
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s, part, nb, out);
+    return hipGetLastError();
+}
+
 template <typename R>
 hipError_t launch_l2(const R* u, long long cs, long long ns, const double* ref, long long npts, int n1, const double* jac,
                      const ElemGeom* geom, double* part, double* out, hipStream_t s) {

@@ -52,6 +52,14 @@ int diag_blocks();
 template <typename R>
 hipError_t launch_diag(const R* u, long long cs, long long ns, long long npts, int n1, const double* jac,
                        const ElemGeom* geom, const ElemMat* mat, double* part, double* out, hipStream_t s);
+// Energy, max |E|, max |H| of a reference-layout FP64 state into out[0..2]
+// (Case::computeTotalEnergy / getMaxE / getMaxH, case.cpp:733-794): one partial
+// triple per block into part (3 energy_blocks(npts) doubles, every one written,
+// none read before), then one reduction in block order.
+constexpr int kEnergyMaxBlocks = 1024;
+int energy_blocks(long long npts);
+hipError_t launch_energy(const double* st, long long S, long long npts, int n1, const double* jac, const double* w,
+                         const double* eps, const double* mu, double* part, double* out, hipStream_t s);
 // Sums of (u - ref)^2 dV and ref^2 dV over all components and nodes into
 // out[0], out[1] (FP64; ref in component blocks of npts).
 template <typename R>

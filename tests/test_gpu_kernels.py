@@ -169,3 +169,46 @@ def test_device_is_gfx950():
     info = nw.device_info(0)
     assert info["count"] >= 1
     assert info["arch"].startswith("gfx950"), info
+
+
+def _diag_call(st, S, o, media, out):
+    d_st, d_jac, d_w = DB.of(st), DB.of(o.jac), DB.of(o.w)
+    eps = DB.of(o.eps) if media else None
+    mu = DB.of(o.mu) if media else None
+    check(lib().nw_launch_diagnostics(d_st.ptr, S, d_jac.ptr, d_w.ptr, eps.ptr if eps else None,
+                                      mu.ptr if mu else None, o.n1, o.nelt, out.ptr, None))
+    check(lib().nw_dev_sync())
+    return out.download(np.float64, 3)
+
+
+@pytest.mark.parametrize("media", [False, True])
+@pytest.mark.parametrize("halo", [False, True])
+def test_diagnostics_launch_matches_oracle(media, halo):
+    """nw_launch_diagnostics (Case::computeTotalEnergy / getMaxE / getMaxH,
+    case.cpp:733-794, on a device state with ghost stride) against the oracle's
+    energy (1e-13 relative: only the summation order differs) and a numpy brute
+    force of max |E|, max |H| over the npts nodes (1e-15 relative)."""
+    o = _mesh(media, halo)
+    S = o.npts + o.nhalo
+    st = _state(o, S)
+    n = o.npts
+    got = _diag_call(st, S, o, media, DB(24))
+    want_e = o.energy(st, S)
+    mag = [np.sqrt(sum(st[c * S:c * S + n] ** 2 for c in cs)).max() for cs in ((0, 1, 2), (3, 4, 5))]
+    assert abs(got[0] - want_e) <= 1e-13 * want_e, (got[0], want_e)
+    assert abs(got[1] - mag[0]) <= 1e-15 * mag[0] and abs(got[2] - mag[1]) <= 1e-15 * mag[1], (got, mag)
+    # distinct E and H: a component-offset slip would swap or mix them
+    assert abs(mag[0] - mag[1]) > 1e-6 * mag[0]
+
+
+def test_diagnostics_launch_tm110_closed_form():
+    """TM110 on the PEC unit cube at t = 0: max |E| = 1 at the nodes x = y = 0.5
+    (element faces of the 4^3 mesh, where GLL end nodes sit), max |H| = 0, and
+    the energy is the continuous 1/8 up to the quadrature error."""
+    o = po.Mesh.box(7, (4, 4, 4), (0, 1, 0, 1, 0, 1))
+    x, y = o.x, o.y
+    z = np.zeros_like(x)
+    st = np.concatenate([z, z, np.sin(np.pi * x) * np.sin(np.pi * y), z, z, z])
+    got = _diag_call(st, o.npts, o, False, DB(24))
+    assert abs(got[1] - 1.0) <= 1e-15 and got[2] == 0.0
+    assert abs(got[0] - 0.125) <= 1e-6, got[0]
