@@ -214,6 +214,28 @@ int nw_dg_segment_option(const nw_dg* s, int seg, int* option, int* matrix_volum
 void nw_dg_destroy(nw_dg* s);
 
 /* ------------------------------------------------------------------------- */
+/* Field snapshots -- replaces Hdf5Writer (src/hdf5_writer.hpp:22-120) as it  */
+/* writes without the HDF5 library (hdf5_writer.cpp:98-119, 121-470,          */
+/* 711-733, 1002-1033)                                                        */
+/* ------------------------------------------------------------------------- */
+typedef struct nw_fields nw_fields;
+/* point fields of each .vtu (FieldSaveOptions, hdf5_writer.hpp:56-67; the
+ * default is E and H; the derived curl/div fields are not offered) */
+enum { NW_SAVE_E = 1, NW_SAVE_H = 2, NW_SAVE_MAGNITUDE_E = 4, NW_SAVE_MAGNITUDE_H = 8, NW_SAVE_ENERGY_DENSITY = 16 };
+/* Hdf5Writer::initialize(h5Path, mesh, enableXdmf, exportContinuous): from
+ * "<dir>/<stem>.h5" it writes <dir>/<stem>.bin (the "NEKWAVE_BIN_V2" archive:
+ * header and node coordinates) and, per saved step, <dir>/<stem>_step_<n>.vtu
+ * plus the <dir>/<stem>.pvd collection.  export_continuous averages coincident
+ * nodes into continuous points (within 1e-6) as the reference does.  The mesh
+ * is read during the call only. */
+int nw_fields_open(const char* h5_path, const nw_mesh* m, int export_continuous, int save_flags, nw_fields** out);
+/* Hdf5Writer::writeStep(step, time, state, npts): state in the uploadState layout,
+ * n = 6 npts. */
+int nw_fields_write(nw_fields* f, int step, double time, const double* state, size_t n);
+/* Hdf5Writer::close + destructor: the final .pvd, the archive closed. */
+void nw_fields_close(nw_fields* f);
+
+/* ------------------------------------------------------------------------- */
 /* Kernel level -- replaces nekwave::device::launch_* (dg_kernels.hpp:13-88) */
 /* Device pointers; stream NULL = default stream.                            */
 /* ------------------------------------------------------------------------- */

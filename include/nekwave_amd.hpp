@@ -195,6 +195,40 @@ class DgSolver {
 using InitialConditionFn = std::function<void(double x, double y, double z, double& Ex, double& Ey, double& Ez,
                                               double& Hx, double& Hy, double& Hz)>;  // case.hpp:21-23
 
+// Hdf5Writer (hdf5_writer.hpp:22-120) as the reference runs it without the
+// HDF5 library: <dir>/<stem>.bin, <stem>_step_<n>.vtu and <stem>.pvd.
+class FieldWriter {
+   public:
+    struct FieldSaveOptions {  // hdf5_writer.hpp:56-67 without the derived curl/div fields
+        bool saveE = true, saveH = true, saveMagnitudeE = false, saveMagnitudeH = false, saveEnergyDensity = false;
+        int flags() const {
+            return (saveE ? NW_SAVE_E : 0) | (saveH ? NW_SAVE_H : 0) | (saveMagnitudeE ? NW_SAVE_MAGNITUDE_E : 0) |
+                   (saveMagnitudeH ? NW_SAVE_MAGNITUDE_H : 0) | (saveEnergyDensity ? NW_SAVE_ENERGY_DENSITY : 0);
+        }
+    };
+    FieldWriter() = default;
+    FieldWriter(const FieldWriter&) = delete;
+    FieldWriter& operator=(const FieldWriter&) = delete;
+    ~FieldWriter() { close(); }
+    void setFieldSaveOptions(const FieldSaveOptions& o) { opt_ = o; }
+    bool initialize(const std::string& h5Path, const Mesh& mesh, bool enableXdmf = true, bool exportContinuous = false) {
+        (void)enableXdmf;  // the XDMF descriptor belongs to the HDF5 archive
+        close();
+        return nw_fields_open(h5Path.c_str(), mesh.handle(), exportContinuous, opt_.flags(), &f_) == NW_OK;
+    }
+    bool writeStep(int step, double time, const double* state, int npts) {
+        return f_ && nw_fields_write(f_, step, time, state, 6 * size_t(npts)) == NW_OK;
+    }
+    void close() {
+        nw_fields_close(f_);
+        f_ = nullptr;
+    }
+
+   private:
+    FieldSaveOptions opt_;
+    nw_fields* f_ = nullptr;
+};
+
 struct Config {
     int order = 3;           // GLL points per direction (config.hpp:24)
     double cfl = -1.0;       // < 0: automatic
@@ -225,6 +259,13 @@ struct Config {
     double packetSigma = 0.30;
     double packetX0 = std::nan("");
     std::string outputDir = "output";  // "none" keeps probe records in memory only
+    // field snapshots (config.hpp:32-48, 193-226): Hdf5Writer's VTU/PVD/.bin files
+    // when exportFields is on and exportFormat is "hdf5" or "vtk"; saveFreq <= 0
+    // means outputFreq
+    bool exportFields = false;
+    std::string exportFormat = "hdf5";
+    int saveFreq = -1;
+    bool exportContinuousVtk = false;
     // the registry entry "wave_type" (Config::set, config.hpp:127-129): the built-in
     // initial condition when Case::setWaveType was not called (case.cpp:163)
     std::string waveType;
@@ -258,6 +299,7 @@ class Case {
     void setWaveType(const std::string& w) { waveType_ = w; }
     // setWaveType's value, else the config's (case.cpp:163)
     const std::string& waveType() const { return waveType_.empty() ? config_.waveType : waveType_; }
+    void setFieldSaveOptions(const FieldWriter::FieldSaveOptions& o) { saveOptions_ = o; }
     void addProbe(double x, double y, double z) { customProbes_.push_back({{x, y, z}}); }
     void addRelativeProbe(double rx, double ry, double rz) { customRel_.push_back({{rx, ry, rz}}); }
 
@@ -365,6 +407,23 @@ class Case {
             solver_->sampleProbes();  // t = 0 (case.cpp:473-474)
             pending.push_back({0, 0.0});
         }
+        // field snapshots at t = 0, every saveFreq steps and at the last step
+        // (case.cpp:438-464, 603-625); the state is downloaded for those steps only
+        writer_.reset();
+        {
+            const std::string& d = config_.outputDir;
+            const bool saveDir = !d.empty() && d != "none" && d != "None" && d != "NONE";
+            if (saveDir && config_.exportFields && (config_.exportFormat == "hdf5" || config_.exportFormat == "vtk")) {
+                ::mkdir(d.c_str(), 0755);
+                writer_.reset(new FieldWriter());
+                writer_->setFieldSaveOptions(saveOptions_);
+                if (!writer_->initialize(d + "/fields.h5", *mesh_, true, config_.exportContinuousVtk))
+                    throw Error(NW_ERR_ARG, nw_last_error());
+                writer_->writeStep(0, 0.0, state_.data(), mesh_->getTotalPoints());
+            }
+        }
+        const int saveFreq = std::max(1, config_.saveFreq > 0 ? config_.saveFreq : outFreq);
+        StateVector snap(writer_ ? state_.size() : 0);
         // The reference's step schedule (case.cpp:562-583); the steps between two
         // outputs with equal dt go to the device as one multi-step launch.
         double chunkDt = 0.0;
@@ -393,6 +452,12 @@ class Case {
             }
             const bool last =
                 (maxSteps > 0 && step >= maxSteps) || (finalTime > 0.0 && currentTime_ >= finalTime - 1e-13);
+            if (writer_ && (step % saveFreq == 0 || last)) {
+                flush();
+                solver_->downloadState(snap.data(), snap.size());
+                if (!writer_->writeStep(step, currentTime_, snap.data(), mesh_->getTotalPoints()))
+                    throw Error(NW_ERR_ARG, nw_last_error());
+            }
             if (step % outFreq == 0 || last) {
                 flush();
                 history_.push_back(solver_->diagnostics());
@@ -406,6 +471,7 @@ class Case {
     }
 
     void postprocess() {
+        writer_.reset();  // case.cpp:689-693
         if (post_) post_(*this);
         if (solver_) solver_->finalize();
     }
@@ -564,6 +630,8 @@ class Case {
     Config config_;
     std::unique_ptr<Mesh> mesh_;
     std::unique_ptr<DgSolver> solver_;
+    std::unique_ptr<FieldWriter> writer_;
+    FieldWriter::FieldSaveOptions saveOptions_;
     StateVector state_;
     InitialConditionFn ic_;
     PostprocessingFn post_;

@@ -23,7 +23,7 @@ import numpy as np
 from . import _lib
 from ._lib import FACE_E, FACE_PEC, FACE_PERIODIC, FACE_PMC, NekWaveError, check, lib
 
-__all__ = ["Mesh", "DgSolver", "BatchSolver", "Case", "Config", "DeviceBuffer", "NekWaveError", "device_info",
+__all__ = ["Mesh", "DgSolver", "BatchSolver", "Case", "Config", "DeviceBuffer", "FieldWriter", "NekWaveError", "device_info",
            "FACE_E", "FACE_PERIODIC", "FACE_PEC", "FACE_PMC"]
 
 
@@ -405,6 +405,51 @@ class DeviceBuffer:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+class FieldWriter:
+    """Hdf5Writer (hdf5_writer.hpp:22-120) as the reference runs it without the
+    HDF5 library: initialize() writes <dir>/<stem>.bin's header, writeStep()
+    appends the step to the archive and writes <stem>_step_<n>.vtu and
+    <stem>.pvd.  Save options: "E", "H", "magnitude_E", "magnitude_H",
+    "energy_density" (the derived curl/div fields are not offered)."""
+
+    FLAGS = {"E": 1, "H": 2, "magnitude_E": 4, "magnitude_H": 8, "energy_density": 16}
+
+    def __init__(self):
+        self._h = None
+        self._flags = self.FLAGS["E"] | self.FLAGS["H"]
+
+    def setFieldSaveOptions(self, fields):
+        self._flags = 0
+        for f in fields:
+            self._flags |= self.FLAGS[f]
+
+    def initialize(self, h5_path, mesh, enable_xdmf=True, export_continuous=False):
+        self.close()
+        h = C.c_void_p()
+        check(lib().nw_fields_open(os.fsencode(h5_path), mesh.handle, int(bool(export_continuous)), self._flags,
+                                   C.byref(h)))
+        self._h = h
+        return True
+
+    def writeStep(self, step, time, state):
+        if self._h is None:
+            raise NekWaveError(_lib.NW_ERR_STATE, "writeStep before initialize")
+        st = np.ascontiguousarray(state, dtype=np.float64)
+        check(lib().nw_fields_write(self._h, int(step), float(time), _dp(st), st.size))
+        return True
+
+    def close(self):
+        if self._h is not None:
+            lib().nw_fields_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
 

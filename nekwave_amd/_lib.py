@@ -96,6 +96,9 @@ SIGNATURES = {
     "nw_launch_restrict_faces": (i, [vp, vp, vp, vp, i64, i64, vp]),
     "nw_launch_compute_flux": (i, [vp] * 9 + [d, i64, i64, vp, vp, vp, vp, i, i64, vp]),
     "nw_launch_add_flux": (i, [vp, vp, vp, vp, i64, i64, vp]),
+    "nw_fields_open": (i, [C.c_char_p, vp, i, i, vp]),
+    "nw_fields_write": (i, [vp, i, C.c_double, vp, C.c_size_t]),
+    "nw_fields_close": (None, [vp]),
     "nw_launch_diagnostics": (i, [vp, i64, vp, vp, vp, vp, i, i, vp, vp]),
     "nw_launch_inv_mass": (i, [vp, vp, vp, i, i, i64, vp, vp, vp]),
     "nw_launch_lsrk45_update": (i, [vp, vp, vp, d, d, d, i64, i64, vp]),

@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "libnekwave_amd.so")
 ARCH = os.environ.get("NW_OFFLOAD_ARCH", "gfx950")
 
 DEVICE_SOURCES = ["nw_fused.hip", "nw_plain.hip"]
-HOST_SOURCES = ["nw_mesh.cpp", "nw_meshio.cpp", "nw_solver.cpp", "nw_api.cpp"]
+HOST_SOURCES = ["nw_mesh.cpp", "nw_meshio.cpp", "nw_fields.cpp", "nw_solver.cpp", "nw_api.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-Wall", "-Wno-unused-result",
           "-I" + os.path.join(ROOT, "include")]

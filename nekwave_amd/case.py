@@ -12,8 +12,15 @@ postprocess(): user hook, then DgSolver.finalize.
 Config keeps the subset of the reference's keys that shape the step
 (config.hpp:21-254): order, cfl, dt, c0/flux_type, nelx/nely/nelz, bounds,
 periodic_*, mesh file, num_steps/max_steps, final_time, output_frequency, probes,
-relative_probes, num_probes, output_dir; plus the extensions precision, pipeline,
+relative_probes, num_probes, output_dir, export_fields, export_format,
+save_frequency, export_continuous_vtk; plus the extensions precision, pipeline,
 jitter, seed.
+
+Field snapshots (case.cpp:438-464, 603-625): with export_fields on and the
+format "hdf5" or "vtk", <output_dir>/fields.bin, fields_step_<n>.vtu and
+fields.pvd as Hdf5Writer writes them without the HDF5 library, at t = 0, every
+save_frequency steps and at the last step; the state is downloaded for those
+steps only.
 
 Probes (probe.cpp:33-216, case.cpp:381-424, 473-474, 594-596) are recorded on the
 device every step; the small history comes back at output frequency and is
@@ -27,7 +34,7 @@ import sys
 
 import numpy as np
 
-from . import DgSolver, Mesh
+from . import DgSolver, FieldWriter, Mesh
 
 
 def _six_sig(v):
@@ -73,7 +80,9 @@ class Config:
                 "output_frequency": "output_frequency", "outputfreq": "output_frequency",
                 "flux": "flux_type", "mesh": "mesh_file", "file": "mesh_file", "path": "mesh_file",
                 "meshfile": "mesh_file", "probe": "probes", "probe_rel": "relative_probes",
-                "outputdir": "output_dir", "sigma": "pulse_sigma"}
+                "outputdir": "output_dir", "sigma": "pulse_sigma", "save_freq": "save_frequency",
+                "export_freq": "save_frequency", "save_format": "export_format", "save_fields": "export_fields",
+                "continuous_vtk": "export_continuous_vtk", "export_continuous": "export_continuous_vtk"}
 
     def __init__(self, **kw):
         self.order = 3
@@ -107,6 +116,13 @@ class Config:
         self.packet_sigma = 0.30
         self.packet_x0 = None
         self.output_dir = "output"
+        # field snapshots (config.hpp:32-48, 193-226): written through the
+        # VTU/PVD/.bin files of Hdf5Writer when export_fields is on and the format
+        # is "hdf5" or "vtk"; save_frequency <= 0 means output_frequency
+        self.export_fields = False
+        self.export_format = "hdf5"
+        self.save_frequency = -1
+        self.export_continuous_vtk = False
         # the registry entry "wave_type" (Config::set, config.hpp:127-129); Case uses it
         # when setWaveType was not called (case.cpp:163)
         self.wave_type = ""
@@ -225,6 +241,14 @@ class Case:
         self.probe_steps = []
         self.probe_times = []
         self.probe_values = np.zeros((0, 0, 6))
+        self.save_fields = ["E", "H"]
+        self.field_writer = None
+
+    def setFieldSaveOptions(self, fields):
+        """Point fields of each snapshot (Case::setFieldSaveOptions with
+        Hdf5Writer::FieldSaveOptions): any of "E", "H", "magnitude_E",
+        "magnitude_H", "energy_density"."""
+        self.save_fields = list(fields)
 
     def setWaveType(self, wave_type):
         """Built-in initial condition when no hook is set (case.cpp:59-62, 164-176,
@@ -407,6 +431,20 @@ class Case:
                                               c.nelx, self.mode_k))
             self.solver.sampleProbes()  # t = 0 (case.cpp:473-474)
             pending.append((0, 0.0))
+        # field snapshots: the t = 0 fields, then every save_frequency steps and at
+        # the last step (case.cpp:438-464, 603-625); the state comes off the
+        # device only for those steps
+        save_dir = c.output_dir and c.output_dir.lower() != "none"
+        writer = None
+        if save_dir and c.export_fields and c.export_format.lower() in ("hdf5", "vtk"):
+            os.makedirs(c.output_dir, exist_ok=True)
+            writer = FieldWriter()
+            writer.setFieldSaveOptions(self.save_fields)
+            writer.initialize(os.path.join(c.output_dir, "fields.h5"), self.mesh, True, c.export_continuous_vtk)
+            writer.writeStep(0, 0.0, self.state)
+        self.field_writer = writer
+        save_freq = max(1, c.save_frequency if c.save_frequency > 0 else out_freq)
+        snap = np.zeros(6 * self.mesh.getTotalPoints()) if writer else None
         # The reference's step schedule (case.cpp:562-583); the steps between two
         # outputs with equal dt go to the device as one multi-step launch with
         # one synchronisation instead of a synchronised step() each.
@@ -429,6 +467,10 @@ class Case:
                 if len(pending) == cap:  # the device history is full: fetch it before the next step
                     flush()
                     self._fetch_probes(pending, out)
+            if writer is not None and (step % save_freq == 0 or final):
+                flush()
+                self.solver.downloadState(snap)
+                writer.writeStep(step, self.current_time, snap)
             if step % out_freq == 0 or final:
                 flush()
                 d = self.solver.diagnostics()
@@ -443,6 +485,9 @@ class Case:
         self.state = self.solver.downloadState()
 
     def postprocess(self):
+        if self.field_writer is not None:  # case.cpp:689-693
+            self.field_writer.close()
+            self.field_writer = None
         if self.post is not None:
             self.post(self)
         if self.solver is not None:

@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "nw_fields.hpp"
 #include "nw_fused.hpp"
 #include "nw_plain.hpp"
 #include "nw_solver.hpp"
@@ -463,6 +464,49 @@ NW_API int nw_launch_diagnostics(const double* st, int64_t S, const double* jac,
     NW_TRY_HIP(nw::launch_energy(st, S, npts, n1, jac, w, eps, mu, part, out, hs(stream)));
     return NW_OK;
     NW_GUARD_END
+}
+
+// ------------------------------------------------------ field snapshots ----
+struct nw_fields {
+    nw::FieldWriter w;
+};
+
+NW_API int nw_fields_open(const char* h5_path, const nw_mesh* m, int export_continuous, int save_flags,
+                          nw_fields** out) {
+    NW_GUARD_BEGIN
+    NW_CHECK_ARG(h5_path && m && out, "fields_open: null argument");
+    NW_CHECK_ARG((save_flags & ~31) == 0, "fields_open: unknown save flag (derived curl/div fields are not offered)");
+    *out = nullptr;
+    nw_fields* f = new nw_fields;
+    if (f->w.open(h5_path, m->m, export_continuous != 0, save_flags)) {
+        delete f;
+        set_error(std::string("fields_open: cannot create the snapshot files for ") + h5_path);
+        return NW_ERR_ARG;
+    }
+    *out = f;
+    return NW_OK;
+    NW_GUARD_END
+}
+
+NW_API int nw_fields_write(nw_fields* f, int step, double time, const double* state, size_t n) {
+    NW_GUARD_BEGIN
+    NW_CHECK_ARG(f && state, "fields_write: null argument");
+    NW_CHECK_ARG(n == size_t(6 * f->w.npts), "fields_write: state size must be 6*npts");
+    if (f->w.write(step, time, state)) {
+        set_error("fields_write: writing the snapshot files failed");
+        return NW_ERR_ARG;
+    }
+    return NW_OK;
+    NW_GUARD_END
+}
+
+NW_API void nw_fields_close(nw_fields* f) {
+    if (!f) return;
+    try {
+        f->w.close();
+    } catch (...) {
+    }
+    delete f;
 }
 
 // ---------------------------------------------------------------- rng ----

@@ -71,3 +71,17 @@ def test_cpp_case_host_checks():
     assert os.path.exists(b.KATS_BIN), "built by __graft_entry__.build()"
     r = subprocess.run([b.KATS_BIN, "host"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0 and "ALL PASSED" in r.stdout, r.stdout + r.stderr
+
+
+def test_field_snapshot_keys():
+    """config.hpp:32-48, 193-226: export_fields / save_fields, export_format /
+    save_format, save_frequency / save_freq / export_freq, export_continuous_vtk /
+    continuous_vtk / export_continuous; save_frequency <= 0 defers to
+    output_frequency (case.cpp:438-440)."""
+    c = Config()
+    assert (c.export_fields, c.export_format, c.save_frequency, c.export_continuous_vtk) == (False, "hdf5", -1, False)
+    f = Config.from_json({"output": {"save_fields": "yes", "save_format": "vtk", "export_freq": 4,
+                                     "continuous_vtk": "on", "output_frequency": 3}})
+    assert (f.export_fields, f.export_format, f.save_frequency, f.export_continuous_vtk) == (True, "vtk", 4, True)
+    g = Config(export_fields=True, save_freq=0)
+    assert g.save_frequency == 0 and g.export_fields
