@@ -33,7 +33,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "DOF-updates/s + % HBM roofline, FP32/batched vs FP64 plain, at stated L2 error"
 HBM_PEAK_GBS = 8000.0
-SIZES = {"fp64": (8, 8), "fp32": (4, 4), "fp32_k64": (4, 8)}
+SIZES = {"fp64": (8, 8), "fp32": (4, 4), "fp32_k64": (4, 8), "bf16x3": (4, 4)}
 
 
 def algorithmic_bytes_per_step(npts, nelt, s_u, s_k):
@@ -251,7 +251,7 @@ def other_config(nw, W, name, steps, error_steps=True):
     wl = W.BUILDERS[name]()
     out = {"nodes": wl.npts, "elements": wl.nelt, "instances": len(wl.meshes), "description": wl.info["description"],
            "dt": wl.dt}
-    precs = ["fp32", "fp64"] if name == "c5" else ["fp32"]
+    precs = {"c5": ["fp32", "fp64"], "c3": ["fp32", "bf16x3"]}.get(name, ["fp32"])
     for prec in precs:
         reps, _, info, segs = time_path(nw, wl, prec, "fused", steps, 2, repeats=3)
         ms = float(np.median(reps))
@@ -271,6 +271,10 @@ def other_config(nw, W, name, steps, error_steps=True):
         s32 = run_state(nw, wl, "fp32", n)
         errs = [s32.relL2(ref[x], x) for x in range(len(wl.meshes))]
         err = {"steps": n, "fp32_vs_fp64_rel_l2": max(errs)}
+        if "bf16x3" in precs:
+            sb = run_state(nw, wl, "bf16x3", n)
+            err["bf16x3_vs_fp64_rel_l2"] = sb.relL2(ref[0], 0)
+            sb.finalize()
         if len(wl.meshes) > 1:
             err["fp32_vs_fp64_rel_l2_median"] = float(np.median(errs))
             an32, an64 = [], []
@@ -301,7 +305,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32_k64", "fp64"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32_k64", "fp64", "bf16x3"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-errors", action="store_true",
                     help="skip the full-length FP32-vs-FP64 runs of the other workloads")
@@ -395,7 +399,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32" if args.precision == "fp32" else ("fp32+fp64k" if args.precision == "fp32_k64" else "fp64"),
+        "dtype": {"fp32": "fp32", "fp32_k64": "fp32+fp64k", "fp64": "fp64", "bf16x3": "fp32+bf16x3"}[args.precision],
         "data": "synthetic",
         "config": {"workload": "periodic [0,1]^3 16^3 hexes, interior vertices jittered +-10% h (seed 2), "
                                "N=7 (n1=8), Gaussian Ez pulse, upwind, LSRK45 auto dt",

@@ -36,8 +36,14 @@ enum {
     NW_ERR_UNSUPPORTED = 6  /* n1/precision/pipeline combination not built */
 };
 
-/* arithmetic of the stage (fields / LSRK register) */
-enum { NW_PREC_FP64 = 0, NW_PREC_FP32 = 1, NW_PREC_FP32_K64 = 2 };
+/* arithmetic of the stage (fields / LSRK register).  NW_PREC_BF16X3: FP32
+ * fields and registers with the volume contractions (dg_kernels.cu:101-124) as
+ * split-precision BF16 products on the matrix cores -- each operand as
+ * hi + lo round-to-nearest BF16 halves, hi*hi + hi*lo + lo*hi accumulated in
+ * FP32 -- fused pipeline, n1 = 8 only.  Its stated error bound is larger than
+ * FP32's (relative L2 against FP64 <= 1e-4 after 1000 steps of the 16^3
+ * workload, against 1e-5 for FP32). */
+enum { NW_PREC_FP64 = 0, NW_PREC_FP32 = 1, NW_PREC_FP32_K64 = 2, NW_PREC_BF16X3 = 3 };
 /* NW_PIPE_PLAIN: the reference's 7-kernel stage (dg_solver.cpp:393-505), FP64 only.
  * NW_PIPE_FUSED: one fused kernel per stage. */
 enum { NW_PIPE_PLAIN = 0, NW_PIPE_FUSED = 1 };

@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "libnekwave_amd.so")
 NW_OK, NW_ERR_ARG, NW_ERR_HIP, NW_ERR_ALLOC, NW_ERR_STATE, NW_ERR_MESH, NW_ERR_UNSUPPORTED = range(7)
 STATUS_NAMES = {0: "NW_OK", 1: "NW_ERR_ARG", 2: "NW_ERR_HIP", 3: "NW_ERR_ALLOC", 4: "NW_ERR_STATE",
                 5: "NW_ERR_MESH", 6: "NW_ERR_UNSUPPORTED"}
-PREC = {"fp64": 0, "fp32": 1, "fp32_k64": 2}
+PREC = {"fp64": 0, "fp32": 1, "fp32_k64": 2, "bf16x3": 3}
 PIPE = {"plain": 0, "fused": 1}
 FACE_E, FACE_PERIODIC, FACE_PEC, FACE_PMC = 0, 1, 2, 3
 NW_MESH_REA_PERIODIC = 1

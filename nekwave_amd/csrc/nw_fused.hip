@@ -167,6 +167,25 @@ __device__ __forceinline__ void st6(__amdgpu_buffer_rsrc_t r, unsigned vo, const
     }
 }
 
+// Split BF16: four FP32 values as round-to-nearest BF16 halves hi = bf16(x),
+// lo = bf16(x - hi), packed as the 4-element operands of the BF16 matrix
+// instructions (x = hi + lo to 2^-17 relative; the product of two such pairs
+// without lo*lo to about 2^-16).
+typedef __attribute__((ext_vector_type(4))) short v4s;
+typedef __attribute__((ext_vector_type(2))) __bf16 v2bf;
+__device__ __forceinline__ void split4(const float* x, v4s& hi, v4s& lo) {
+    unsigned h[2], l[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const v2bf a = {(__bf16)x[2 * t], (__bf16)x[2 * t + 1]};
+        const v2bf b = {(__bf16)(x[2 * t] - (float)a[0]), (__bf16)(x[2 * t + 1] - (float)a[1])};
+        h[t] = __builtin_bit_cast(unsigned, a);
+        l[t] = __builtin_bit_cast(unsigned, b);
+    }
+    hi = __builtin_bit_cast(v4s, (unsigned long long)h[0] | ((unsigned long long)h[1] << 32));
+    lo = __builtin_bit_cast(v4s, (unsigned long long)l[0] | ((unsigned long long)l[1] << 32));
+}
+
 // 1/J of the FP64 Jacobian: in FP32 the hardware reciprocal (1 ulp) of the
 // rounded determinant; in FP64 the IEEE quotient.
 __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -199,7 +218,7 @@ __device__ __forceinline__ void norm_and_inverse(double n2, double& len, double&
 //   4. lift: each node gathers its (at most three) face terms, one per
 //      direction, in fixed face order; inverse mass; LSRK update; stores;
 //      the next group's geometry.
-template <int N1, int EPW, bool FIRST, typename R, typename KR, bool MX>
+template <int N1, int EPW, bool FIRST, typename R, typename KR, int MX>
 __global__ __launch_bounds__(EPW* N1* N1* N1) __attribute__((amdgpu_waves_per_eu(WavesPerEU<EPW * N1 * N1 * N1, sizeof(R)>::value)))
 void k_fused_stage(FusedArgs A) {
     constexpr int N2 = N1 * N1, NP = N1 * N2, NF = 6 * N2, NT = EPW * NP;
@@ -212,7 +231,8 @@ void k_fused_stage(FusedArgs A) {
     };
     static_assert(sizeof(ElemRec) == ERW * sizeof(double), "element record layout");
     // MX: the volume contractions on the matrix cores (n1 = 8, FP32 fields,
-    // one element per workgroup), else on the vector ALUs.
+    // one element per workgroup) -- 1: FP32 products, 2: split BF16 products
+    // (hi*hi + hi*lo + lo*hi with FP32 accumulation) -- else (0) on the vector ALUs.
     static_assert(!MX || (N1 == 8 && EPW == 1 && sizeof(R) == 4), "matrix-core volume shape");
     __shared__ R sU[EPW][6][NP];
     __shared__ R sF[EPW * NF + 1][6];  // face terms dA*F; the last row stays zero
@@ -507,6 +527,12 @@ void k_fused_stage(FusedArgs A) {
             R Ar[N1];
 #pragma unroll
             for (int q = 0; q < N1; ++q) Ar[q] = sD[(4 * h + (lane & 3)) * N1 + q];
+            // split BF16: the lane's row of D as hi and lo halves, q = 0..3 and 4..7
+            v4s Ahi[2], Alo[2];
+            if constexpr (MX == 2) {
+#pragma unroll
+                for (int kh = 0; kh < 2; ++kh) split4(&Ar[4 * kh], Ahi[kh], Alo[kh]);
+            }
             const int p0 = wv < 6 ? 2 * wv : 12 + 3 * (wv - 6), np = wv < 6 ? 2 : 3;
             for (int pp = 0; pp < np; ++pp) {
                 const int P = p0 + pp, d = P / 6, c = P - 6 * d;
@@ -524,10 +550,28 @@ void k_fused_stage(FusedArgs A) {
                     for (int q = 0; q < N1; ++q) { B0[q] = u[l0 + 64 * q]; B1[q] = u[l1 + 64 * q]; }
                 }
                 v4f a0 = {0.0f, 0.0f, 0.0f, 0.0f}, a1 = {0.0f, 0.0f, 0.0f, 0.0f};
+                if constexpr (MX == 1) {
 #pragma unroll
-                for (int q = 0; q < N1; ++q) {
-                    a0 = __builtin_amdgcn_mfma_f32_4x4x1f32(Ar[q], B0[q], a0, 0, 0, 0);
-                    a1 = __builtin_amdgcn_mfma_f32_4x4x1f32(Ar[q], B1[q], a1, 0, 0, 0);
+                    for (int q = 0; q < N1; ++q) {
+                        a0 = __builtin_amdgcn_mfma_f32_4x4x1f32(Ar[q], B0[q], a0, 0, 0, 0);
+                        a1 = __builtin_amdgcn_mfma_f32_4x4x1f32(Ar[q], B1[q], a1, 0, 0, 0);
+                    }
+                } else {
+                    // v_mfma_f32_4x4x4_16b_bf16: lane 4b + r holds row r of block b's
+                    // A (k = 0..3), lane 4b + j column j of its B, the same lane
+                    // placement as the FP32 form; the small products first
+#pragma unroll
+                    for (int kh = 0; kh < 2; ++kh) {
+                        v4s Bh0, Bl0, Bh1, Bl1;
+                        split4(&B0[4 * kh], Bh0, Bl0);
+                        split4(&B1[4 * kh], Bh1, Bl1);
+                        a0 = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(Alo[kh], Bh0, a0, 0, 0, 0);
+                        a1 = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(Alo[kh], Bh1, a1, 0, 0, 0);
+                        a0 = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(Ahi[kh], Bl0, a0, 0, 0, 0);
+                        a1 = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(Ahi[kh], Bl1, a1, 0, 0, 0);
+                        a0 = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(Ahi[kh], Bh0, a0, 0, 0, 0);
+                        a1 = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(Ahi[kh], Bh1, a1, 0, 0, 0);
+                    }
                 }
                 if (d < 2) {
                     *reinterpret_cast<v4f*>(&sA[d][c][8 * l0 + 4 * h]) = a0;
@@ -696,6 +740,10 @@ void k_fused_stage(FusedArgs A) {
 constexpr int kEpw[kMaxN1 + 1][2] = {
     {1, 1}, {1, 1}, {8, 32}, {9, 4}, {4, 1}, {2, 1}, {1, 3}, {1, 2}, {1, 1}, {1, 1}, {1, 1}};
 constexpr bool matrix_volume(int n1, int opt, int rsz) { return n1 == 8 && opt == 1 && rsz == 4; }
+// volume contraction mode of a launch: 0 vector ALUs, 1 FP32 matrix cores, 2 split BF16 matrix cores
+constexpr int volume_mode(int n1, int opt, int rsz, bool bf16x3) {
+    return bf16x3 ? 2 : (matrix_volume(n1, opt, rsz) ? 1 : 0);
+}
 
 constexpr int lds_per_elem(int n1, int rsz) { return (6 * n1 * n1 * n1 + 36 * n1 * n1) * rsz; }
 constexpr int epw_of(int n1, int opt, int rsz) {
@@ -717,10 +765,10 @@ int resident_groups(K kernel, int threads) {
     return (per_cu > 0 ? per_cu : 1) * cus;
 }
 
-template <int N1, int OPT, typename R, typename KR>
+template <int N1, int OPT, typename R, typename KR, bool BX = false>
 hipError_t launch_one(const FusedArgs& a, hipStream_t s) {
-    constexpr int EPW = epw_of(N1, OPT, sizeof(R));
-    constexpr bool MX = matrix_volume(N1, OPT, sizeof(R));
+    constexpr int EPW = BX ? 1 : epw_of(N1, OPT, sizeof(R));
+    constexpr int MX = volume_mode(N1, OPT, sizeof(R), BX);
     constexpr int NT = EPW * N1 * N1 * N1;
     static int resident = 0;
     if (resident <= 0) resident = resident_groups(k_fused_stage<N1, EPW, false, R, KR, MX>, NT);
@@ -769,20 +817,25 @@ LaunchFn pick_n1(int n1, int option) {
 
 int fused_epw(int n1, int option, int prec) {
     if (n1 < 0 || n1 > kMaxN1) return 0;
+    if (prec == kBF16X3) return 1;
     return epw_of(n1, option ? 1 : 0, prec == kFP64 ? 8 : 4);
 }
 
 bool fused_matrix_volume(int n1, int option, int prec) {
-    return matrix_volume(n1, option ? 1 : 0, prec == kFP64 ? 8 : 4);
+    return prec == kBF16X3 || matrix_volume(n1, option ? 1 : 0, prec == kFP64 ? 8 : 4);
 }
 
-bool fused_supported(int n1, int prec) { return n1 >= 2 && n1 <= 9 && prec >= 0 && prec < kNumPrec; }
+bool fused_supported(int n1, int prec) {
+    if (prec == kBF16X3) return n1 == 8;  // the 4x4x4 BF16 tiles cover 8-point lines
+    return n1 >= 2 && n1 <= 9 && prec >= 0 && prec < kNumPrec;
+}
 
 hipError_t launch_fused(int n1, int prec, int option, const FusedArgs& a, hipStream_t s) {
     LaunchFn fn = nullptr;
     if (prec == kFP64) fn = pick_n1<double, double>(n1, option);
     else if (prec == kFP32) fn = pick_n1<float, float>(n1, option);
     else if (prec == kFP32K64) fn = pick_n1<float, double>(n1, option);
+    else if (prec == kBF16X3 && n1 == 8) fn = &launch_one<8, 0, float, float, true>;  // one shape: both options
     if (!fn) return hipErrorInvalidValue;
     if (a.nelt <= 0) return hipSuccess;
     return fn(a, s);

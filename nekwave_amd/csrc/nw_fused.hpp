@@ -5,7 +5,9 @@
 
 namespace nw {
 
-enum Prec { kFP64 = 0, kFP32 = 1, kFP32K64 = 2, kNumPrec = 3 };
+// kBF16X3: FP32 fields and registers, volume contractions as split BF16
+// products on the matrix cores (n1 = 8 only).
+enum Prec { kFP64 = 0, kFP32 = 1, kFP32K64 = 2, kBF16X3 = 3, kNumPrec = 4 };
 
 // Everything one launch of the fused stage kernel needs.  One launch covers a
 // contiguous range of elements that share n1 (a whole mesh, or one n1 segment

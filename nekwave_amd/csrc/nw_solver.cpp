@@ -222,7 +222,7 @@ int Solver::init(const Mesh* const* meshes, int count, double c0_, const nw_opts
     stride = npts + nhalo;
 
     const size_t rsz = (prec == kFP64) ? 8 : 4;
-    const size_t ksz = (prec == kFP32) ? 4 : 8;
+    const size_t ksz = (prec == kFP32 || prec == kBF16X3) ? 4 : 8;
     if ((st = dalloc(reinterpret_cast<void**>(&d_dt), 8))) return st;
     if ((st = dalloc(reinterpret_cast<void**>(&d_diag_part), sizeof(double) * 3 * diag_blocks()))) return st;
     if ((st = dalloc(reinterpret_cast<void**>(&d_diag_out), sizeof(double) * 3))) return st;

@@ -84,6 +84,17 @@ def test_c2_fp32_vs_fp64_after_1000_steps(c2, precision, bound):
     assert err <= bound, err
 
 
+def test_c2_bf16x3_vs_fp64_after_1000_steps(c2):
+    """The split-BF16 mode's stated bound: relative L2 <= 1e-4 against the FP64
+    path after 1000 steps of the 16^3 workload."""
+    wl, o, _ = c2
+    ref = product_run(wl.meshes[0], wl.states[0], wl.dt, 1000, "fp64", "fused")
+    got = product_run(wl.meshes[0], wl.states[0], wl.dt, 1000, "bf16x3", "fused")
+    err = rel_l2(an.dV(o), got, ref)
+    print("c2 bf16x3 vs fp64 after 1000 steps relL2=%.3e" % err, flush=True)
+    assert err <= 1e-4, err
+
+
 def _c3_oracle(wl):
     o = po.Mesh.box(8, (32, 32, 32), (0, 1, 0, 1, 0, 1))
     o.set_materials(1.0 + 3.0 * W.uniforms(3, 32 ** 3), np.ones(32 ** 3))
@@ -130,6 +141,10 @@ def test_c3_fp32_vs_fp64_full_length():
     err = device_rel_l2(wl, wl.steps, "fp32")[0]
     print("c3 fp32 vs fp64 after %d steps relL2=%.3e" % (wl.steps, err), flush=True)
     assert err <= 1e-5, err
+    # the workload's named mode: split-BF16 contractions, within their 1e-4 bound
+    errb = device_rel_l2(wl, wl.steps, "bf16x3")[0]
+    print("c3 bf16x3 vs fp64 after %d steps relL2=%.3e" % (wl.steps, errb), flush=True)
+    assert errb <= 1e-4, errb
 
 
 def test_c3_upwind_energy_nonincreasing_fp32():

@@ -70,6 +70,35 @@ def test_fp32_matches_oracle(name, precision):
     assert err <= 2e-5, err
 
 
+@pytest.mark.parametrize("name", ["periodic_jitter_p7", "media_pec_p7"])
+def test_bf16x3_matches_oracle_within_its_bound(name):
+    """Split-BF16 matrix-core contractions (NW_PREC_BF16X3): each operand as
+    hi + lo BF16 halves, hi*hi + hi*lo + lo*hi in FP32 -- a product error of
+    about 2^-16 relative instead of FP32's 2^-24.  After 20 steps within 2e-4 of
+    the FP64 oracle (FP32: 2e-5), and measurably different from the FP32 path
+    (the split products are really used); deterministic and graph = stream."""
+    m, o, c0 = _build(name)
+    st0 = gaussian_state(o, width=0.2)
+    dt = 0.5 * o.auto_dt()
+    ref = run_oracle(o, st0, dt, 20, c0)
+    got = run_product(m, st0, dt, 20, c0, "bf16x3", "fused")
+    f32 = run_product(m, st0, dt, 20, c0, "fp32", "fused")
+    err, err32 = rel_l2(o, got, ref), rel_l2(o, f32, ref)
+    print("%s bf16x3 relL2=%.3e (fp32 %.3e)" % (name, err, err32))
+    assert err <= 2e-4, err
+    assert not np.array_equal(got, f32)
+    again = run_product(m, st0, dt, 20, c0, "bf16x3", "fused", use_graph=False)
+    assert np.array_equal(got, again)
+
+
+def test_bf16x3_needs_n1_8():
+    m, o, c0 = _build("pec_tm110_p4")
+    s = nw.DgSolver(precision="bf16x3", pipeline="fused")
+    with pytest.raises(nw.NekWaveError) as e:
+        s.initialize(m, c0)
+    assert e.value.status == nw._lib.NW_ERR_UNSUPPORTED
+
+
 def test_graph_and_stream_paths_are_bitwise_equal_and_deterministic():
     m, o, c0 = _build("periodic_jitter_p3")
     st0 = random_state(o, 1)
