@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <numeric>
 #include <vector>
@@ -658,6 +659,26 @@ int Solver::build_graph(int parity) {
     return NW_OK;
 }
 
+// Tuned tile options kept for the life of the process, keyed by what decides
+// them: device, n1, precision, element count of the segment and whether it is a
+// single mesh (Morton walk) or a batch segment (storage order).  A later solver
+// of the same shape takes the choice without timing again.
+namespace {
+struct TuneKey {
+    int device, n1, prec, single;
+    long long nelt;
+    bool operator<(const TuneKey& o) const {
+        if (device != o.device) return device < o.device;
+        if (n1 != o.n1) return n1 < o.n1;
+        if (prec != o.prec) return prec < o.prec;
+        if (single != o.single) return single < o.single;
+        return nelt < o.nelt;
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, int> g_tune;
+}  // namespace
+
 // Autotuner: per n1 segment, time both tile options on a regular stage (it
 // reads k and writes u_out and k, all rewritten by the first real stage, which
 // does not read k) and keep the faster.  The options alternate twice and each
@@ -667,6 +688,15 @@ int Solver::tune() {
     if (pipe != NW_PIPE_FUSED) return NW_OK;
     NW_TRY_HIP(hipSetDevice(device));
     for (Segment& sg : seg) {
+        const TuneKey key{device, sg.n1, prec, inst.size() == 1 ? 1 : 0, sg.nelt};
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tune.find(key);
+            if (it != g_tune.end()) {
+                sg.option = it->second;
+                continue;
+            }
+        }
         float best[2] = {1e30f, 1e30f};
         for (int opt = 0; opt < 2; ++opt) {  // warm both (first launch sets up the kernel)
             const int st = launch_segment(sg, 1, cur, opt, stream);
@@ -686,6 +716,8 @@ int Solver::tune() {
                 best[opt] = std::min(best[opt], ms);
             }
         sg.option = best[1] < best[0] ? 1 : 0;
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[key] = sg.option;
     }
     tuned = true;
     return NW_OK;
