@@ -296,6 +296,17 @@ int nw_launch_unpack_halo(const double* recvBuf, double* state, const int* recvB
 int nw_launch_diagnostics(const double* state, int64_t stateStride, const double* jac, const double* w,
                           const double* eps, const double* mu, int n1, int nelt, double* out, void* stream);
 
+/* No counterpart in the reference: nw_launch_diagnostics for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results, bit
+ * for bit; the partial sums live in a fixed 24 KiB array per GPU (1024 partial
+ * triples, held until the process ends) and the call allocates nothing.  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (other
+ * streams, other host threads) race.  NW_ERR_ARG if the state would need more
+ * partial sums than the array holds. */
+int nw_launch_diagnostics_noalloc(const double* state, int64_t stateStride, const double* jac, const double* w,
+                                  const double* eps, const double* mu, int n1, int nelt, double* out, void* stream);
+
 /* Seeded synthetic inputs: std::mt19937_64(seed) with uniform = (x >> 11) 2^-53,
  * normal = Box-Muller first variate of (1 - uniform, uniform); n draws. */
 int nw_rand_uniform(uint64_t seed, size_t n, double* out);

@@ -100,6 +100,7 @@ SIGNATURES = {
     "nw_fields_write": (i, [vp, i, C.c_double, vp, C.c_size_t]),
     "nw_fields_close": (None, [vp]),
     "nw_launch_diagnostics": (i, [vp, i64, vp, vp, vp, vp, i, i, vp, vp]),
+    "nw_launch_diagnostics_noalloc": (i, [vp, i64, vp, vp, vp, vp, i, i, vp, vp]),
     "nw_launch_inv_mass": (i, [vp, vp, vp, i, i, i64, vp, vp, vp]),
     "nw_launch_lsrk45_update": (i, [vp, vp, vp, d, d, d, i64, i64, vp]),
     "nw_launch_pack_halo": (i, [vp, vp, vp, vp, vp, i, i64, vp]),

@@ -466,6 +466,29 @@ NW_API int nw_launch_diagnostics(const double* st, int64_t S, const double* jac,
     NW_GUARD_END
 }
 
+// The same reduction with its partial sums in the library's fixed per-device
+// array (nw_plain.hip `scratch`): no allocation of any kind in the call.
+NW_API int nw_launch_diagnostics_noalloc(const double* st, int64_t S, const double* jac, const double* w,
+                                         const double* eps, const double* mu, int n1, int nelt, double* out,
+                                         void* stream) {
+    int rc = check_diag_args(st, S, jac, w, n1, nelt, out);
+    if (rc) return rc;
+    const long long npts = (long long)nelt * n1 * n1 * n1;
+    const int nb = nw::energy_blocks(npts);
+    if (nb > nw::kEnergyScratchBlocks) {
+        set_error("diagnostics_noalloc: " + std::to_string(nb) + " partial sums exceed the scratch's " +
+                  std::to_string(nw::kEnergyScratchBlocks));
+        return NW_ERR_ARG;
+    }
+    double* part = nullptr;
+    if (nw::energy_scratch(&part) != 0) {
+        set_error("diagnostics_noalloc: no scratch address for the current device");
+        return NW_ERR_HIP;
+    }
+    NW_TRY_HIP(nw::launch_energy(st, S, npts, n1, jac, w, eps, mu, part, out, hs(stream)));
+    return NW_OK;
+}
+
 // ------------------------------------------------------ field snapshots ----
 struct nw_fields {
     nw::FieldWriter w;

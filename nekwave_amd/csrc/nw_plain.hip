@@ -10,6 +10,8 @@
 //   * lift: one thread per node gathering its face contributions in face order
 //     0..5 (deterministic; the reference uses FP64 atomics, dg_kernels.cu:329);
 //   * elementwise kernels are grid-stride loops sized for 256 CUs.
+#include <mutex>
+
 #include "nw_plain.hpp"
 
 #include "nw_fused.hpp"
@@ -712,6 +714,29 @@ hipError_t launch_diag(const R* u, long long cs, long long ns, long long npts, i
     hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s, part, nb, out);
     return hipGetLastError();
 }
+// Partial sums of nw_launch_diagnostics_noalloc: a fixed device array (one
+// copy per GPU, each at its own address, held until the process ends) instead
+// of an allocation.  Host code must use the address energy_scratch() returns,
+// never this name.  Every call on a GPU shares that GPU's copy: calls that
+// overlap on one GPU race.
+static __device__ __align__(256) double scratch[3 * kEnergyScratchBlocks];
+static_assert(kEnergyScratchBlocks >= kEnergyMaxBlocks, "the scratch holds every block of the largest grid");
+
+int energy_scratch(double** out) {
+    static std::mutex mu;
+    static double* addr[kMaxDevices] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return -1;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -2;
+        addr[dev] = static_cast<double*>(p);
+    }
+    *out = addr[dev];
+    return 0;
+}
+
 int energy_blocks(long long npts) {
     const long long b = (npts + kBlock - 1) / kBlock;
     return static_cast<int>(b < 1 ? 1 : (b > kEnergyMaxBlocks ? kEnergyMaxBlocks : b));

@@ -58,6 +58,12 @@ hipError_t launch_diag(const R* u, long long cs, long long ns, long long npts, i
 // none read before), then one reduction in block order.
 constexpr int kEnergyMaxBlocks = 1024;
 int energy_blocks(long long npts);
+// The device address of this GPU's fixed partial-sum array (3 kEnergyScratchBlocks
+// doubles, 24 KiB): looked up once per device with that device current; 0 on
+// success, negative when the lookup failed.
+constexpr int kEnergyScratchBlocks = 1024;
+constexpr int kMaxDevices = 64;
+int energy_scratch(double** out);
 hipError_t launch_energy(const double* st, long long S, long long npts, int n1, const double* jac, const double* w,
                          const double* eps, const double* mu, double* part, double* out, hipStream_t s);
 // Sums of (u - ref)^2 dV and ref^2 dV over all components and nodes into
