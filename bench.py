@@ -155,12 +155,23 @@ def load_traffic(name):
 
 
 def profile_run(nw, W, precision, steps, workload="c2"):
+    """A few steps without graphs, so a kernel trace sees every launch: tile
+    option 0 for a single mesh; for the batched cavities the tuned option of each
+    n1 segment (one persistent launch per segment and stage)."""
     import ctypes
 
     wl = getattr(W, workload)()
-    s = nw.DgSolver(precision=precision, pipeline="fused", use_graph=False, tile_option=0)
-    s.initialize(wl.meshes[0], wl.c0)
-    s.uploadState(wl.states[0])
+    if len(wl.meshes) == 1:
+        s = nw.DgSolver(precision=precision, pipeline="fused", use_graph=False, tile_option=0)
+        s.initialize(wl.meshes[0], wl.c0)
+        s.uploadState(wl.states[0])
+    else:
+        s = nw.BatchSolver(precision=precision, use_graph=False)
+        s.initialize(wl.meshes, wl.c0)
+        for x, st in enumerate(wl.states):
+            s.uploadState(x, st)
+        s.tune()
+        print("segments:", s.segments(), flush=True)
     s.run(wl.dt, steps)
     s.finalize()
     ms = ctypes.c_double()
@@ -311,7 +322,7 @@ def main():
                     help="skip the full-length FP32-vs-FP64 runs of the other workloads")
     ap.add_argument("--accuracy-steps", type=int, default=1000)
     ap.add_argument("--configs", default="c3,c4,c5", help="other workloads to time (comma list, or 'none')")
-    ap.add_argument("--profile-workload", default="c2", choices=["c2", "c3", "c5"],
+    ap.add_argument("--profile-workload", default="c2", choices=["c2", "c3", "c4", "c5"],
                     help="workload of --profile-run (single-mesh workloads)")
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for rocprofv3 counter passes: the fused stage on the workload "

@@ -202,6 +202,41 @@ __device__ __forceinline__ void norm_and_inverse(double n2, double& len, double&
     inv = len > 1e-14 ? 1.0 / len : 0.0;
 }
 
+// FP64 geometry at reference point (r, s, t) of a trilinear element: the
+// cofactors C = J J^-T (mesh.cpp:291-309 times J) and 1/J, rounded to R.  On an
+// affine element (c4..c7 exactly 0) every product with them is an exact zero,
+// so the result is the same bits at every node.
+template <typename R>
+__device__ __forceinline__ void node_geometry(const ElemGeom* G, double r, double s, double t, R (&C)[10]) {
+    const Tangents T = tangents(G, r, s, t);
+    double Cr[3], Cs[3], Ct[3];
+    cross3(T.s, T.t, Cr);  // (rx, ry, rz) * J
+    cross3(T.t, T.r, Cs);  // (sx, sy, sz) * J
+    cross3(T.r, T.s, Ct);  // (tx, ty, tz) * J
+    const double det = T.r[0] * Cr[0] + T.r[1] * Cr[1] + T.r[2] * Cr[2];
+    C[0] = R(Cr[0]); C[1] = R(Cr[1]); C[2] = R(Cr[2]);
+    C[3] = R(Cs[0]); C[4] = R(Cs[1]); C[5] = R(Cs[2]);
+    C[6] = R(Ct[0]); C[7] = R(Ct[1]); C[8] = R(Ct[2]);
+    C[9] = recip(R(det));
+}
+
+// Records of an affine segment: each element's geometry computed once (at the
+// reference origin, equal to every node's) into the first ten words of its
+// 26-word record, as doubles holding the R values exactly.
+template <typename R>
+__global__ void k_affine_records(double* rec, int nelt) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nelt) return;
+    double* q = rec + size_t(e) * 26;
+    ElemGeom g;
+#pragma unroll
+    for (int x = 0; x < 24; ++x) (&g.c[0][0])[x] = q[x];
+    R C[10];
+    node_geometry(&g, 0.0, 0.0, 0.0, C);
+#pragma unroll
+    for (int x = 0; x < 10; ++x) q[x] = double(C[x]);
+}
+
 // Persistent grid: in iteration t workgroup W takes the group at position
 // t * gstride + W of the group order, so one iteration of the grid covers a
 // contiguous window of the order (a compact block of elements when the order
@@ -218,7 +253,8 @@ __device__ __forceinline__ void norm_and_inverse(double n2, double& len, double&
 //   4. lift: each node gathers its (at most three) face terms, one per
 //      direction, in fixed face order; inverse mass; LSRK update; stores;
 //      the next group's geometry.
-template <int N1, int EPW, bool FIRST, typename R, typename KR, int MX>
+// AFF: the segment is affine and its records hold the geometry (FusedArgs::affine).
+template <int N1, int EPW, bool FIRST, typename R, typename KR, int MX, bool AFF>
 __global__ __launch_bounds__(EPW* N1* N1* N1) __attribute__((amdgpu_waves_per_eu(WavesPerEU<EPW * N1 * N1 * N1, sizeof(R)>::value)))
 void k_fused_stage(FusedArgs A) {
     constexpr int N2 = N1 * N1, NP = N1 * N2, NF = 6 * N2, NT = EPW * NP;
@@ -240,6 +276,9 @@ void k_fused_stage(FusedArgs A) {
     __shared__ R sC[EPW * NF + SPARE][9];  // per face point: Nanson vector, minus-side trace (6)
     __shared__ R sD[N2];               // D, row-major: sD[i * N1 + l] = D_il
     __shared__ ElemRec sE[2][EPW];     // element records, double-buffered over groups
+    // the i = n1-1 plane of the last slot of the previous group (its pre-stage
+    // fields), rows of 7 words so a wave's reads of consecutive rows do not conflict
+    __shared__ R sP3[N2][7];
     // n1 = 8, 9, FP32, vector-ALU volume: the fields again with the s-lines
     // (i, k) and t-lines (i, j) contiguous, lines 12 words apart (16-byte
     // aligned; at n1 = 8 eight lines cover all 32 banks), so those directions
@@ -260,9 +299,12 @@ void k_fused_stage(FusedArgs A) {
     // window, and with it most face neighbours, stays in that XCD's L2.
     const int hb = blockIdx.x;
     const int W = (hb & 7) * A.wchunk + (hb >> 3);
-    if (W >= A.ngroups) return;  // uniform over the block, before any barrier
-    const int WT = A.gstride;
-    const int tl = (A.ngroups - 1 - W) / WT;  // this workgroup's last iteration
+    if (W >= A.nseq) return;  // uniform over the block, before any barrier
+    // this workgroup's groups: gorder[s0 .. s0 + tl], in that order
+    typedef const __attribute__((address_space(4))) int CInt;
+    const int s0 = ((CInt*)A.gseq)[W];
+    const int tl = ((CInt*)A.gseq)[W + 1] - s0 - 1;  // last iteration
+    if (tl < 0) return;
 
     const int tid = threadIdx.x;
     const int es = tid / NP;
@@ -303,11 +345,10 @@ void k_fused_stage(FusedArgs A) {
     // They are issued at the top of an iteration and first waited on at its
     // first barrier, before which the iteration reads no LDS.
     constexpr bool SF = N2 == 64 && EPW == 1;
-    typedef const __attribute__((address_space(4))) int CInt;
     const int wface = min(__builtin_amdgcn_readfirstlane(tid >> 6), 5);  // SF: this wave's face
     CInt* const sGo = (CInt*)A.gorder;
     CInt* const sNb = (CInt*)A.fnb;
-    auto order_s = [&](int t) -> int { return sGo[min(t, tl) * WT + W]; };
+    auto order_s = [&](int t) -> int { return sGo[s0 + min(t, tl)]; };
     auto node_of = [&](int g) -> unsigned {  // clamped: every read stays in bounds
         return static_cast<unsigned>(A.node_base + (long long)clamp_e(g, es) * NP + m);
     };
@@ -327,10 +368,18 @@ void k_fused_stage(FusedArgs A) {
     struct Rec {
         double w[GQ];
     };
+    // Where a face point's plus-side trace comes from (an x face -- face 1 or
+    // 3 -- whose neighbour element is already in LDS needs no gather):
+    //   kGather: the gathered registers up[t];
+    //   kSlot:   the neighbouring slot of this group (face 1 <-> slot + 1, face 3 <-> slot - 1);
+    //   kCarry:  face 3 of slot 0, whose neighbour is the last slot of this
+    //            workgroup's previous group: its i = n1-1 plane, kept in sP3;
+    //   kMirror: PEC mirror face, no plus side.
+    enum { kGather = 0, kSlot = 1, kCarry = 2, kMirror = 3 };
     struct Pre {
         R u[6];          // this node's fields
-        R up[FPT][6];    // plus-side trace of each face point (unused on mirror faces)
-        bool mirror[FPT];
+        R up[FPT][6];    // plus-side trace of each face point (kGather only)
+        int kind[FPT];
     };
     auto fetch_nbr = [&](int g, Nbr& N) {
 #pragma unroll
@@ -361,20 +410,37 @@ void k_fused_stage(FusedArgs A) {
             if (x < EPW * ERW) (&sE[buf][0].g.c[0][0])[x] = Q.w[t];
         }
     };
-    auto fetch = [&](int g, const Nbr& N, Pre& P) {
+    // gprev: the group this workgroup processes just before g (-1: none)
+    auto fetch = [&](int g, const Nbr& N, Pre& P, int gprev) {
         const unsigned nd = node_of(g);
         ld6(rU, nd * UB, P.u);
-        // issued by every thread (threads without a face point read their own
-        // node): the same number of loads on every path keeps the compiler's
-        // memory-counter waits exact instead of conservative
+        // issued by every thread (threads without a face point, and x faces
+        // served from LDS, read their own node): the same number of loads on
+        // every path keeps the compiler's memory-counter waits exact instead of
+        // conservative
 #pragma unroll
         for (int t = 0; t < FPT; ++t) {
             const int fp = tid + t * NT;
+            const int efs = min(fp / NF, EPW - 1);
             const int r = fp - (fp / NF) * NF;
-            const int pq = r - (r / N2) * N2, p = pq % N1, q = pq / N1;
+            const int f = r / N2;
+            const int pq = r - f * N2, p = pq % N1, q = pq / N1;
             const int4 b = N.b[t];
-            P.mirror[t] = b.x < 0;
-            const unsigned idx = (b.x >= 0 && fp < EPW * NF) ? unsigned(b.x + p * b.y + q * b.z) : nd;
+            int kind = b.x < 0 ? kMirror : kGather;
+            // an x face paired with the opposite x face of a neighbour in the
+            // same orientation (plus node of (p, q) = base + p n1 + q n1^2)
+            // (node indices fit in 32 bits: the launcher checks the byte offsets)
+            if (fp < EPW * NF && b.y == N1 && b.z == N2) {
+                const unsigned nb = unsigned(A.node_base), bx = unsigned(b.x);
+                if (f == 1 && efs + 1 < EPW && bx == nb + unsigned(clamp_e(g, efs + 1)) * NP) kind = kSlot;
+                if (f == 3 && efs > 0 && bx == nb + unsigned(clamp_e(g, efs - 1)) * NP + (N1 - 1)) kind = kSlot;
+                if (f == 3 && efs == 0 && gprev >= 0 && bx == nb + unsigned(clamp_e(gprev, EPW - 1)) * NP + (N1 - 1))
+                    kind = kCarry;
+            }
+            // at n1 = 8 with one element per workgroup a wave holds one face: uniform
+            if constexpr (SF) kind = __builtin_amdgcn_readfirstlane(kind);
+            P.kind[t] = kind;
+            const unsigned idx = (kind == kGather && fp < EPW * NF) ? unsigned(b.x + p * b.y + q * b.z) : nd;
             ld6(rU, idx * UB, P.up[t]);
         }
     };
@@ -394,7 +460,7 @@ void k_fused_stage(FusedArgs A) {
         fetch_nbr(gB, nb1);
         fetch_rec(gB, rec1);
         put_rec(rec0, 0);
-        fetch(gA, nb0, cur);
+        fetch(gA, nb0, cur, -1);
     }
     // Wait for the prologue's reads here: left in flight, the compiler's
     // memory-counter analysis merges them into the loop and then waits, at the
@@ -407,17 +473,16 @@ void k_fused_stage(FusedArgs A) {
     // C = J J^-T (mesh.cpp:291-309 times J) and 1/J, rounded to R.  A group's
     // geometry is computed at the end of the previous iteration, where the
     // waves otherwise wait for the prefetched fields to land.
+    // Affine segments (every element a parallelepiped) carry these ten values,
+    // precomputed once per element by k_affine_records with the same
+    // arithmetic, in the record's first ten words: read them instead.
     auto geometry = [&](const ElemRec& ER, R(&C)[10]) {
-        const Tangents T = tangents(&ER.g, zr, zs, zt);
-        double Cr[3], Cs[3], Ct[3];
-        cross3(T.s, T.t, Cr);  // (rx, ry, rz) * J
-        cross3(T.t, T.r, Cs);  // (sx, sy, sz) * J
-        cross3(T.r, T.s, Ct);  // (tx, ty, tz) * J
-        const double det = T.r[0] * Cr[0] + T.r[1] * Cr[1] + T.r[2] * Cr[2];
-        C[0] = R(Cr[0]); C[1] = R(Cr[1]); C[2] = R(Cr[2]);
-        C[3] = R(Cs[0]); C[4] = R(Cs[1]); C[5] = R(Cs[2]);
-        C[6] = R(Ct[0]); C[7] = R(Ct[1]); C[8] = R(Ct[2]);
-        C[9] = recip(R(det));
+        if constexpr (AFF) {
+#pragma unroll
+            for (int x = 0; x < 10; ++x) C[x] = R((&ER.g.c[0][0])[x]);
+        } else {
+            node_geometry(&ER.g, zr, zs, zt, C);
+        }
     };
     R geo[10];
     geometry(sE[0][es], geo);
@@ -507,7 +572,7 @@ void k_fused_stage(FusedArgs A) {
         // iteration, which every wave of the CU issues at once, stays short
         // and the barrier comes sooner; they still land before the iteration
         // ends (and in flight while this group computes).
-        fetch(gB, nb1, nxt);
+        fetch(gB, nb1, nxt, gA);
 
         // ---- 2. volume: reference derivatives (local_grad3, dg_kernels.cu:101-124)
         // folded straight into J * curl (dg_kernels.cu:140-149) ----
@@ -627,14 +692,26 @@ void k_fused_stage(FusedArgs A) {
                 const int f = r / N2, pq = r - f * N2, p = pq % N1, q = pq / N1;
                 const R* mine = &sC[fp][3];
                 R jE[3], jH[3];
-                if (cur.mirror[t]) {  // PEC mirror: [[E]] = -2 E-, [[H]] = 0 (dg_kernels.cu:244-251)
+                const int kind = cur.kind[t];
+                if (kind == kMirror) {  // PEC mirror: [[E]] = -2 E-, [[H]] = 0 (dg_kernels.cu:244-251)
 #pragma unroll
                     for (int c = 0; c < 3; ++c) { jE[c] = R(-2) * mine[c]; jH[c] = R(0); }
                 } else {
+                    // plus trace: kSlot -- face 1: node (0, p, q) of slot + 1, face 3:
+                    // node (n1-1, p, q) of slot - 1; kCarry -- the kept plane
+                    const R* lp = &sP3[pq][0];
+                    int ls = 1;
+                    if constexpr (EPW > 1) {
+                        if (kind == kSlot) {
+                            lp = &sU[f == 1 ? efs + 1 : efs - 1][0][(f == 1 ? 0 : N1 - 1) + N1 * p + N2 * q];
+                            ls = NP;
+                        }
+                    }
+                    const bool lds = kind != kGather;
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
-                        jE[c] = cur.up[t][c] - mine[c];
-                        jH[c] = cur.up[t][c + 3] - mine[c + 3];
+                        jE[c] = (lds ? lp[c * ls] : cur.up[t][c]) - mine[c];
+                        jH[c] = (lds ? lp[(c + 3) * ls] : cur.up[t][c + 3]) - mine[c + 3];
                     }
                 }
                 // unit normal and area element (mesh.cpp:1032-1038)
@@ -711,6 +788,12 @@ void k_fused_stage(FusedArgs A) {
         // so they store the same values to the same places (no branch: a
         // store on one path only would make the compiler's memory-counter
         // waits conservative and stall on the stores).
+        // the last slot's i = n1-1 plane (pre-stage fields) for the next group's
+        // face 3; its readers in this iteration are past the barrier above
+        if (es == EPW - 1 && i == N1 - 1) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) sP3[j + N1 * k][c] = cur.u[c];
+        }
         {
             const R sE_ = R(ER.ieps) * iJ, sH_ = R(ER.imu) * iJ;
             KR kn[6];
@@ -765,33 +848,69 @@ int resident_groups(K kernel, int threads) {
     return (per_cu > 0 ? per_cu : 1) * cus;
 }
 
+// Workgroups of one kernel shape resident on the whole device at once.
+template <int N1, int OPT, typename R, typename KR, bool BX = false>
+int resident_of() {
+    constexpr int EPW = BX ? 1 : epw_of(N1, OPT, sizeof(R));
+    constexpr int MX = volume_mode(N1, OPT, sizeof(R), BX);
+    constexpr int NT = EPW * N1 * N1 * N1;
+    static int resident = 0;
+    if (resident <= 0) resident = resident_groups(k_fused_stage<N1, EPW, false, R, KR, MX, false>, NT);
+    return resident;
+}
+
 template <int N1, int OPT, typename R, typename KR, bool BX = false>
 hipError_t launch_one(const FusedArgs& a, hipStream_t s) {
     constexpr int EPW = BX ? 1 : epw_of(N1, OPT, sizeof(R));
     constexpr int MX = volume_mode(N1, OPT, sizeof(R), BX);
     constexpr int NT = EPW * N1 * N1 * N1;
-    static int resident = 0;
-    if (resident <= 0) resident = resident_groups(k_fused_stage<N1, EPW, false, R, KR, MX>, NT);
-    if (resident <= 0) return hipErrorInvalidValue;
+    if (resident_of<N1, OPT, R, KR, BX>() <= 0) return hipErrorInvalidValue;
     // every byte offset (component block included) of a field, register or
     // ghost slot fits in 32 bits
     const unsigned long long span = 6ull * (unsigned long long)(a.stride > a.kstride ? a.stride : a.kstride);
     if (span * sizeof(double) >= (1ull << 32)) return hipErrorInvalidValue;
     FusedArgs b = a;
     b.ngroups = (a.nelt + EPW - 1) / EPW;
-    if (!a.gorder) return hipErrorInvalidValue;
-    const int want = b.ngroups < resident ? b.ngroups : resident;
-    b.wchunk = (want + 7) / 8;
-    b.gstride = 8 * b.wchunk;  // workgroups past ngroups return at once
+    if (!a.gorder || !a.gseq || a.nseq <= 0 || a.nseq > b.ngroups) return hipErrorInvalidValue;
+    b.wchunk = (a.nseq + 7) / 8;
+    b.gstride = 8 * b.wchunk;  // workgroups past nseq return at once
     const dim3 grid(b.gstride), block(NT);
-    if (a.first_stage)
-        hipLaunchKernelGGL((k_fused_stage<N1, EPW, true, R, KR, MX>), grid, block, 0, s, b);
-    else
-        hipLaunchKernelGGL((k_fused_stage<N1, EPW, false, R, KR, MX>), grid, block, 0, s, b);
+    if (a.affine) {
+        if (a.first_stage)
+            hipLaunchKernelGGL((k_fused_stage<N1, EPW, true, R, KR, MX, true>), grid, block, 0, s, b);
+        else
+            hipLaunchKernelGGL((k_fused_stage<N1, EPW, false, R, KR, MX, true>), grid, block, 0, s, b);
+    } else {
+        if (a.first_stage)
+            hipLaunchKernelGGL((k_fused_stage<N1, EPW, true, R, KR, MX, false>), grid, block, 0, s, b);
+        else
+            hipLaunchKernelGGL((k_fused_stage<N1, EPW, false, R, KR, MX, false>), grid, block, 0, s, b);
+    }
     return hipGetLastError();
 }
 
 using LaunchFn = hipError_t (*)(const FusedArgs&, hipStream_t);
+using ResidentFn = int (*)();
+
+template <int N1, typename R, typename KR>
+ResidentFn pick_resident(int option) {
+    return option == 0 ? &resident_of<N1, 0, R, KR> : &resident_of<N1, 1, R, KR>;
+}
+
+template <typename R, typename KR>
+ResidentFn pick_resident_n1(int n1, int option) {
+    switch (n1) {
+        case 2: return pick_resident<2, R, KR>(option);
+        case 3: return pick_resident<3, R, KR>(option);
+        case 4: return pick_resident<4, R, KR>(option);
+        case 5: return pick_resident<5, R, KR>(option);
+        case 6: return pick_resident<6, R, KR>(option);
+        case 7: return pick_resident<7, R, KR>(option);
+        case 8: return pick_resident<8, R, KR>(option);
+        case 9: return pick_resident<9, R, KR>(option);
+        default: return nullptr;
+    }
+}
 
 template <int N1, typename R, typename KR>
 LaunchFn pick(int option) {
@@ -830,6 +949,15 @@ bool fused_supported(int n1, int prec) {
     return n1 >= 2 && n1 <= 9 && prec >= 0 && prec < kNumPrec;
 }
 
+int fused_resident(int n1, int prec, int option) {
+    ResidentFn fn = nullptr;
+    if (prec == kFP64) fn = pick_resident_n1<double, double>(n1, option);
+    else if (prec == kFP32) fn = pick_resident_n1<float, float>(n1, option);
+    else if (prec == kFP32K64) fn = pick_resident_n1<float, double>(n1, option);
+    else if (prec == kBF16X3 && n1 == 8) fn = &resident_of<8, 0, float, float, true>;
+    return fn ? fn() : 0;
+}
+
 hipError_t launch_fused(int n1, int prec, int option, const FusedArgs& a, hipStream_t s) {
     LaunchFn fn = nullptr;
     if (prec == kFP64) fn = pick_n1<double, double>(n1, option);
@@ -839,6 +967,16 @@ hipError_t launch_fused(int n1, int prec, int option, const FusedArgs& a, hipStr
     if (!fn) return hipErrorInvalidValue;
     if (a.nelt <= 0) return hipSuccess;
     return fn(a, s);
+}
+
+hipError_t launch_affine_records(int prec, double* rec, int nelt, hipStream_t s) {
+    if (nelt <= 0) return hipSuccess;
+    const dim3 grid((nelt + 255) / 256), block(256);
+    if (prec == kFP64)
+        hipLaunchKernelGGL(k_affine_records<double>, grid, block, 0, s, rec, nelt);
+    else
+        hipLaunchKernelGGL(k_affine_records<float>, grid, block, 0, s, rec, nelt);
+    return hipGetLastError();
 }
 
 int make_face_nbr(const int* fcode, int nelt, int n1, long long node_base, long long ghost_base, int elem_base,

@@ -25,7 +25,9 @@ struct FusedArgs {
     int ngroups;              // element groups (EPW elements each); set by the launcher
     int gstride;              // workgroups of the persistent grid; set by the launcher
     int wchunk;               // workgroups per XCD (XCD-aware remap); set by the launcher
-    const int* gorder;        // ngroups entries: the order groups are visited in (a permutation)
+    const int* gorder;        // ngroups entries: the groups in visiting order (a permutation)
+    const int* gseq;          // nseq + 1 offsets into gorder: workgroup W visits gorder[gseq[W] .. gseq[W+1])
+    int nseq;                 // workgroup sequences (<= the resident workgroups, fused_resident)
     const ElemGeom* geom;     // per element, segment-local
     const double* rec;        // per element, segment-local: 24 trilinear coefficients, 1/eps, 1/mu
     const ElemMat* mat;       // per element, global index (always present; vacuum = 1)
@@ -36,15 +38,23 @@ struct FusedArgs {
     int first_stage;          // a == 0: k is not read
     const void* fw;           // per element face (nelt*6, segment-local), 4 R: flux weights
                               // (wEh, wEe, wHe, wHh), see face_flux_weights
+    int affine;               // every element affine: records hold the geometry (launch_affine_records)
 };
 
 // Elements per workgroup options (index 0/1) for a given n1.
 int fused_epw(int n1, int option, int prec);
+// Workgroups of that shape resident on the current device at once (0: not built).
+int fused_resident(int n1, int prec, int option);
 // Whether tile option `option` computes the volume contractions on the matrix cores.
 bool fused_matrix_volume(int n1, int option, int prec);
 bool fused_supported(int n1, int prec);
 hipError_t launch_fused(int n1, int prec, int option, const FusedArgs& a, hipStream_t s);
 int fused_upload_tables(const GllTables& t);
+// Rewrites the records (26 doubles per element) of an affine segment so their
+// first ten words hold the element's geometry as the fused kernel computes it
+// per node (cofactors, 1/J in the precision's R); the kernel then reads them
+// (FusedArgs::affine).
+hipError_t launch_affine_records(int prec, double* rec, int nelt, hipStream_t s);
 
 // Face orientation: (p, q) on a face -> (p', q') on the neighbour face.
 // bit0: swap p,q; bit1: p' = n1-1-p'; bit2: q' = n1-1-q'.

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -43,45 +44,130 @@ static const double kRkB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.
                                2277821191437.0 / 14882151754819.0};
 
 // Visiting order of the element groups (EPW consecutive elements each) of one
-// fused-kernel launch.  For a single mesh: a Morton (Z-order) curve over the
-// groups' first-element centroids, so the window of groups the persistent grid
-// covers in one iteration is a compact block of elements and most face
-// neighbours are read while still in L2.  For a batch (m == nullptr) the
-// storage order: instances are small and independent.
-static void append_group_order(const Mesh* m, int epw, int nelt, std::vector<int>& out) {
+// fused-kernel launch, as `nseq` workgroup sequences: workgroup W visits
+// order[seq[W] .. seq[W+1]) one group per iteration.
+//
+// Chains: runs of groups g, g+1, ... where the last element of g meets the
+// first element of g+1 through its face 1 in the same orientation (the x-lines
+// of a box mesh).  Along a chain the kernel takes the face-3 traces of a group
+// from the plane its predecessor left in LDS instead of gathering them, so when
+// the mesh has chains every workgroup walks along them: equal chains are cut
+// into equal segments dealt round-robin in Morton (Z-order) order of their
+// first group's centroid; chains of different lengths (a batch: storage order)
+// are concatenated and cut into nseq pieces of nearly equal length, neighbouring
+// pieces on one XCD.  Without chains (single groups), the grid's iteration t
+// covers positions t nseq .. t nseq + nseq - 1 of the Morton curve, a compact
+// block of elements whose face neighbours are read while still in L2.
+static bool uniform_chains(const std::vector<int>& cstart, int nchain) {
+    for (int c = 1; c < nchain; ++c)
+        if (cstart[c + 1] - cstart[c] != cstart[1] - cstart[0]) return false;
+    return nchain > 0;
+}
+
+static void append_walk(const Mesh* m, const int4* fnb, int n1, int epw, int nelt, long long node_base, int nseq,
+                        std::vector<int>& order, std::vector<int>& seq) {
     const int ng = (nelt + epw - 1) / epw;
-    const size_t base = out.size();
-    out.resize(base + size_t(ng));
-    std::iota(out.begin() + base, out.end(), 0);
-    if (!m || ng < 2) return;
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (int g = 0; g < ng; ++g)
-        for (int d = 0; d < 3; ++d) {
-            const double x = m->geom[size_t(g) * epw].c[d][0];  // centroid of the trilinear map
-            lo[d] = std::min(lo[d], x);
-            hi[d] = std::max(hi[d], x);
-        }
-    auto spread = [](uint64_t v) {  // 10 bits -> every third bit
-        v &= 0x3ff;
-        v = (v | (v << 16)) & 0x030000ffull;
-        v = (v | (v << 8)) & 0x0300f00full;
-        v = (v | (v << 4)) & 0x030c30c3ull;
-        v = (v | (v << 2)) & 0x09249249ull;
-        return v;
-    };
-    std::vector<uint64_t> key(static_cast<size_t>(ng));
+    const long long np = (long long)n1 * n1 * n1;
+    nseq = std::max(1, std::min(nseq, ng));
+    const int obase = static_cast<int>(order.size());
+    auto first = [&](int g) { return g * epw; };
+    auto last = [&](int g) { return std::min(g * epw + epw - 1, nelt - 1); };
+    // chains
+    std::vector<int> cstart;
     for (int g = 0; g < ng; ++g) {
-        uint64_t code = 0;
-        for (int d = 0; d < 3; ++d) {
-            const double span = hi[d] - lo[d];
-            const double x = span > 0 ? (m->geom[size_t(g) * epw].c[d][0] - lo[d]) / span : 0.0;
-            const uint64_t q = static_cast<uint64_t>(std::min(1023.0, std::max(0.0, x * 1024.0)));
-            code |= spread(q) << d;
+        bool joins = false;
+        if (g > 0) {
+            const int4 b = fnb[size_t(last(g - 1)) * 6 + 1];
+            joins = b.y == n1 && b.z == n1 * n1 && (long long)b.x == node_base + first(g) * np;
         }
-        key[size_t(g)] = (code << 32) | static_cast<uint64_t>(g);
+        if (!joins) cstart.push_back(g);
     }
-    std::sort(key.begin(), key.end());
-    for (int g = 0; g < ng; ++g) out[base + size_t(g)] = static_cast<int>(key[size_t(g)] & 0xffffffffull);
+    const int nchain = static_cast<int>(cstart.size());
+    cstart.push_back(ng);
+    // Morton key of a group's first element centroid (single mesh)
+    std::vector<uint64_t> key;
+    if (m) {
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int g = 0; g < ng; ++g)
+            for (int d = 0; d < 3; ++d) {
+                const double x = m->geom[size_t(first(g))].c[d][0];
+                lo[d] = std::min(lo[d], x);
+                hi[d] = std::max(hi[d], x);
+            }
+        auto spread = [](uint64_t v) {  // 10 bits -> every third bit
+            v &= 0x3ff;
+            v = (v | (v << 16)) & 0x030000ffull;
+            v = (v | (v << 8)) & 0x0300f00full;
+            v = (v | (v << 4)) & 0x030c30c3ull;
+            v = (v | (v << 2)) & 0x09249249ull;
+            return v;
+        };
+        key.resize(size_t(ng));
+        for (int g = 0; g < ng; ++g) {
+            uint64_t code = 0;
+            for (int d = 0; d < 3; ++d) {
+                const double span = hi[d] - lo[d];
+                const double x = span > 0 ? (m->geom[size_t(first(g))].c[d][0] - lo[d]) / span : 0.0;
+                const uint64_t q = static_cast<uint64_t>(std::min(1023.0, std::max(0.0, x * 1024.0)));
+                code |= spread(q) << d;
+            }
+            key[size_t(g)] = code;
+        }
+    }
+    if (2 * nchain > ng) {
+        // (almost) no chains: round-robin positions of the Morton curve (storage order for a batch)
+        std::vector<int> curve(static_cast<size_t>(ng));
+        std::iota(curve.begin(), curve.end(), 0);
+        if (m) std::stable_sort(curve.begin(), curve.end(), [&](int a, int b) { return key[size_t(a)] < key[size_t(b)]; });
+        for (int W = 0; W < nseq; ++W) {
+            seq.push_back(static_cast<int>(order.size()) - obase);
+            for (long long p = W; p < ng; p += nseq) order.push_back(curve[size_t(p)]);
+        }
+    } else if (uniform_chains(cstart, nchain)) {
+        // equal chains (the x-lines of a box mesh): each cut into m equal segments,
+        // m the smallest divisor of the chain length giving every workgroup a
+        // segment; segments in Morton order dealt round-robin, so in every round
+        // the workgroups of one XCD walk neighbouring segments side by side and
+        // the y and z neighbours of a group are visited at the same time
+        const int lc = cstart[1] - cstart[0];
+        int mseg = lc;
+        for (int d = 1; d <= lc; ++d)
+            if (lc % d == 0 && (long long)nchain * d >= nseq) {
+                mseg = d;
+                break;
+            }
+        const int ls = lc / mseg;
+        std::vector<int> sgs;  // first group of each segment
+        for (int c = 0; c < nchain; ++c)
+            for (int x = 0; x < mseg; ++x) sgs.push_back(cstart[c] + x * ls);
+        if (m) std::stable_sort(sgs.begin(), sgs.end(), [&](int a, int b) { return key[size_t(a)] < key[size_t(b)]; });
+        const int nsg = static_cast<int>(sgs.size());
+        nseq = std::min(nseq, nsg);
+        for (int W = 0; W < nseq; ++W) {
+            seq.push_back(static_cast<int>(order.size()) - obase);
+            for (int x = W; x < nsg; x += nseq)
+                for (int g = sgs[size_t(x)]; g < sgs[size_t(x)] + ls; ++g) order.push_back(g);
+        }
+    } else {
+        std::vector<int> co(static_cast<size_t>(nchain));
+        std::iota(co.begin(), co.end(), 0);
+        if (m)
+            std::stable_sort(co.begin(), co.end(),
+                             [&](int a, int b) { return key[size_t(cstart[a])] < key[size_t(cstart[b])]; });
+        std::vector<int> stream;
+        stream.reserve(static_cast<size_t>(ng));
+        for (int c : co)
+            for (int g = cstart[c]; g < cstart[c + 1]; ++g) stream.push_back(g);
+        const int base = ng / nseq, extra = ng % nseq;
+        size_t at = 0;
+        for (int W = 0; W < nseq; ++W) {
+            seq.push_back(static_cast<int>(order.size()) - obase);
+            const int len = base + (W < extra ? 1 : 0);
+            order.insert(order.end(), stream.begin() + at, stream.begin() + at + len);
+            at += static_cast<size_t>(len);
+        }
+    }
+    seq.push_back(static_cast<int>(order.size()) - obase);
 }
 
 unsigned long long mesh_signature(const Mesh& m) {
@@ -209,9 +295,11 @@ int Solver::init(const Mesh* const* meshes, int count, double c0_, const nw_opts
             sg.elem_base = I.elem_base;
             sg.node_base = I.node_base;
             sg.option = opt;
+            sg.affine = true;
             seg.push_back(sg);
         }
         seg.back().nelt += m.nelt;
+        seg.back().affine = seg.back().affine && m.all_affine;
     }
     nelt = elems;
     npts = nodes;
@@ -246,6 +334,16 @@ int Solver::init(const Mesh* const* meshes, int count, double c0_, const nw_opts
             rec[size_t(e) * 26 + 25] = mat[e].imu;
         }
         if ((st = upload_vec(&d_rec, rec))) return st;
+        // NW_GENERAL_GEOMETRY=1 keeps the per-node geometry on affine meshes too
+        // (the two give the same bits; tests compare them)
+        const char* gen = std::getenv("NW_GENERAL_GEOMETRY");
+        const bool per_node = gen && gen[0] == '1';
+        if (pipe == NW_PIPE_FUSED)
+            for (Segment& sg : seg) {
+                if (per_node) sg.affine = false;
+                if (sg.affine)
+                    NW_TRY_HIP(launch_affine_records(prec, d_rec + size_t(sg.elem_base) * 26, sg.nelt, stream));
+            }
     }
 
     // halo stand-in metadata (dg_solver.cpp:131-170 with one neighbour: self)
@@ -287,13 +385,24 @@ int Solver::init(const Mesh* const* meshes, int count, double c0_, const nw_opts
             }
         }
         if ((st = upload_vec(&d_fnb, fnb))) return st;
-        std::vector<int> gorder;
+        std::vector<int> gorder, gseq;
         for (int opt = 0; opt < 2; ++opt)
             for (Segment& sg : seg) {
+                const int epw = fused_epw(sg.n1, opt, prec);
+                const int resident = fused_resident(sg.n1, prec, opt);
+                if (resident <= 0) {
+                    set_error("no resident workgroups for the fused kernel");
+                    return NW_ERR_HIP;
+                }
                 sg.goff[opt] = static_cast<long long>(gorder.size());
-                append_group_order(count == 1 ? meshes[0] : nullptr, fused_epw(sg.n1, opt, prec), sg.nelt, gorder);
+                sg.soff[opt] = static_cast<long long>(gseq.size());
+                const size_t before = gseq.size();
+                append_walk(count == 1 ? meshes[0] : nullptr, fnb.data() + size_t(sg.elem_base) * 6, sg.n1, epw,
+                            sg.nelt, sg.node_base, resident, gorder, gseq);
+                sg.nseq[opt] = static_cast<int>(gseq.size() - before) - 1;
             }
         if ((st = upload_vec(&d_gorder, gorder))) return st;
+        if ((st = upload_vec(&d_gseq, gseq))) return st;
         // flux weights per element face from the two media (the plus side of a
         // mirror face is the element itself; of a cut face, its ghost medium)
         std::vector<double> w64(size_t(nelt) * 24);
@@ -441,11 +550,14 @@ int Solver::launch_segment(const Segment& sg, int s, int parity, int option, hip
     a.gmat = d_gmat;
     a.fnb = d_fnb + size_t(sg.elem_base) * 6;
     a.gorder = d_gorder + sg.goff[option ? 1 : 0];
+    a.gseq = d_gseq + sg.soff[option ? 1 : 0];
+    a.nseq = sg.nseq[option ? 1 : 0];
     a.dt = d_dt;
     a.a = kRkA[s];
     a.b = kRkB[s];
     a.c0 = c0;
     a.first_stage = (s == 0);
+    a.affine = sg.affine ? 1 : 0;
     a.fw = static_cast<const char*>(d_fw) + size_t(sg.elem_base) * 24 * (prec == kFP64 ? 8 : 4);
     NW_TRY_HIP(launch_fused(sg.n1, prec, option, a, st));
     return NW_OK;

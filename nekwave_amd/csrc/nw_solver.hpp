@@ -18,8 +18,11 @@ struct Instance {
 // A run of elements sharing n1: one fused launch per stage.
 struct Segment {
     int n1 = 0, nelt = 0, elem_base = 0, option = 0;
+    bool affine = false;  // every element a parallelepiped: records hold the geometry
     long long node_base = 0;
     long long goff[2] = {0, 0};  // offset of this segment's group order, per tile option
+    long long soff[2] = {0, 0};  // offset of its workgroup sequence offsets, per tile option
+    int nseq[2] = {0, 0};        // its workgroup sequences, per tile option
 };
 
 // FNV-1a hash of what makes a mesh this mesh: n1, corner coordinates (bit
@@ -60,6 +63,7 @@ struct Solver {
     void* d_fw = nullptr;  // flux weights per element face, 4 values of the field type
     double* d_rec = nullptr;  // per element: 24 trilinear coefficients, 1/eps, 1/mu
     int* d_gorder = nullptr;  // per segment and tile option: the order the fused kernel visits groups in
+    int* d_gseq = nullptr;    // per segment and tile option: workgroup sequence offsets into it
     // plain pipeline (reference residency, single mesh)
     double *d_state = nullptr, *d_kk = nullptr, *d_rhs = nullptr, *d_D = nullptr, *d_w3 = nullptr;
     double* d_met[13] = {};  // x y z jac rx sx tx ry sy ty rz sz tz

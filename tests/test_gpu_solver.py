@@ -15,6 +15,7 @@ import pytest
 
 import analytic as an
 import nekwave_amd as nw
+import pyoracle as po
 from gpu_util import gaussian_state, pair, random_state, rel_l2, run_oracle, run_product
 
 pytestmark = pytest.mark.gpu
@@ -97,6 +98,41 @@ def test_bf16x3_needs_n1_8():
     with pytest.raises(nw.NekWaveError) as e:
         s.initialize(m, c0)
     assert e.value.status == nw._lib.NW_ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("name", ["pec_tm110_p4", "media_pec_p5", "halo_media_p4", "sheared_n8"])
+def test_affine_records_equal_per_node_geometry(name, monkeypatch):
+    """On meshes of parallelepipeds the fused kernel reads each element's
+    geometry from its record (computed once per element) instead of per node;
+    the bits must be those of the per-node computation (NW_GENERAL_GEOMETRY=1)."""
+    if name == "sheared_n8":  # n1 = 8 (the matrix-core shapes), parallelepipeds that are not boxes
+        corners = []
+        for e in range(4):
+            for a in range(8):
+                i, j, k = a in (1, 2, 5, 6), a in (2, 3, 6, 7), a >= 4
+                x, y, z = e + i, 0.5 * j, 0.75 * k
+                corners.append((x + 0.3 * y + 0.1 * z, y + 0.2 * z, z))
+        corners = np.array(corners).reshape(4, 8, 3)
+        kind = np.full((4, 6), nw.FACE_PEC, np.int32)
+        ne = np.full((4, 6), -1, np.int32)
+        nf = np.full((4, 6), -1, np.int32)
+        for e in range(3):
+            kind[e, 1] = kind[e + 1, 3] = nw.FACE_E
+            ne[e, 1], nf[e, 1], ne[e + 1, 3], nf[e + 1, 3] = e + 1, 3, e, 1
+        m = nw.Mesh.fromCorners(8, corners, kind, ne, nf)
+        o = po.Mesh.from_corners(8, corners, kind, ne, nf)
+        c0 = 1.0
+    else:
+        m, o, c0 = _build(name)
+    assert m.info().all_affine
+    st0 = random_state(o, 9)
+    dt = 0.5 * o.auto_dt()
+    for prec in ("fp64", "fp32", "fp32_k64") + (("bf16x3",) if m.getN() == 8 else ()):
+        monkeypatch.delenv("NW_GENERAL_GEOMETRY", raising=False)
+        a = run_product(m, st0, dt, 3, c0, prec, "fused")
+        monkeypatch.setenv("NW_GENERAL_GEOMETRY", "1")
+        b = run_product(m, st0, dt, 3, c0, prec, "fused")
+        assert np.array_equal(a, b), prec
 
 
 def test_graph_and_stream_paths_are_bitwise_equal_and_deterministic():
