@@ -81,6 +81,12 @@ def rel_l2(a, b, w):
 def host_threads():
     """The host cores this process may use, as `nproc` counts them: OMP_NUM_THREADS
     when set, else the CPUs of the process's affinity mask."""
+    import subprocess
+
+    try:
+        return int(subprocess.run(["nproc"], capture_output=True, text=True, timeout=10).stdout.strip())
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
     env = os.environ.get("OMP_NUM_THREADS", "")
     if env.strip().isdigit() and int(env) > 0:
         return int(env)
@@ -144,6 +150,7 @@ def cpu_baseline(W, dt, budget_s=12.0):
             "sample": "%d LSRK45 step(s) of the same 16^3 n1=8 jittered periodic mesh, FP64, %d OpenMP "
                       "threads (%s), %.1f s" % (n, threads, cpu_model(), t),
             "value_1_thread": 6 * o.npts / t_one, "cpu_model": cpu_model(), "os_cpu_count": os.cpu_count(),
+            "nproc": threads, "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
             "c1_full_run": c1_out}
 
 

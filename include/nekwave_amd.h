@@ -136,7 +136,7 @@ typedef struct nw_opts {
     int pipeline;     /* NW_PIPE_* (default FUSED) */
     int use_graph;    /* capture each step as a hipGraph (default 1) */
     int device;       /* HIP device ordinal (default 0) */
-    int tile_option;  /* fused tile option 0/1 (elements per workgroup; at n1 = 8 in FP32,
+    int tile_option;  /* fused tile option 0/1/2 (elements per workgroup; at n1 = 8 in FP32,
                        * 1 = matrix-core volume), -1 = tuned default */
 } nw_opts;
 void nw_opts_default(nw_opts* o);
@@ -207,12 +207,12 @@ int nw_dg_probe_history(nw_dg* s, double* out, int max_records, int* nrecords);
 /* Host half of ProbeManager::init (probe.cpp:57-96): element and clamped (r,s,t)
  * of each probe. */
 int nw_mesh_locate_probes(const nw_mesh* m, int n, const double* xyz, int* elem, double* rst);
-/* Autotuner: time both tile options per segment and keep the faster (runs by
- * itself before the first step when tile_option = -1). */
+/* Autotuner: time every distinct tile option per segment and keep the fastest
+ * (runs by itself before the first step when tile_option = -1). */
 int nw_dg_tune(nw_dg* s);
 /* n1, element count and chosen elements per workgroup of segment `seg`. */
 int nw_dg_segment_info(const nw_dg* s, int seg, int* n1, int* nelt, int* epw);
-/* Extension: the tile option segment `seg` runs (0/1, as tuned or set) and
+/* Extension: the tile option segment `seg` runs (0/1/2, as tuned or set) and
  * whether that option computes the volume contractions on the matrix cores
  * (n1 = 8 with FP32 fields, option 1; results equal the vector path's). */
 int nw_dg_segment_option(const nw_dg* s, int seg, int* option, int* matrix_volume);

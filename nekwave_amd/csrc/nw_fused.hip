@@ -817,11 +817,13 @@ void k_fused_stage(FusedArgs A) {
     }
 }
 
-// Elements per workgroup, two options per n1 (index 0 is the default), capped
-// so one workgroup's LDS stays within 64 KiB (two or more workgroups per CU).
-// At n1 = 8 option 1 is instead the matrix-core volume for FP32 fields.
-constexpr int kEpw[kMaxN1 + 1][2] = {
-    {1, 1}, {1, 1}, {8, 32}, {9, 4}, {4, 1}, {2, 1}, {1, 3}, {1, 2}, {1, 1}, {1, 1}, {1, 1}};
+// Elements per workgroup, three options per n1 (index 0 is the default),
+// capped so one workgroup's LDS stays within 64 KiB (two or more workgroups per
+// CU).  At n1 = 8 option 1 is instead the matrix-core volume for FP32 fields.
+// An option whose shape repeats an earlier one is that option (fused_option_of).
+constexpr int kEpw[kMaxN1 + 1][kNumTileOptions] = {
+    {1, 1, 1}, {1, 1, 1}, {8, 32, 16}, {9, 4, 9}, {4, 1, 2}, {2, 1, 4}, {1, 3, 2}, {1, 2, 1}, {1, 1, 1}, {1, 1, 1},
+    {1, 1, 1}};
 constexpr bool matrix_volume(int n1, int opt, int rsz) { return n1 == 8 && opt == 1 && rsz == 4; }
 // volume contraction mode of a launch: 0 vector ALUs, 1 FP32 matrix cores, 2 split BF16 matrix cores
 constexpr int volume_mode(int n1, int opt, int rsz, bool bf16x3) {
@@ -835,6 +837,14 @@ constexpr int epw_of(int n1, int opt, int rsz) {
     const int cap_thr = 1024 / (n1 * n1 * n1);
     const int cap = cap_lds < cap_thr ? cap_lds : cap_thr;
     return want < cap ? want : (cap > 0 ? cap : 1);
+}
+// The option that builds the same kernel as `opt` (itself unless it repeats an
+// earlier option's elements per workgroup and volume mode).
+constexpr int canonical_option(int n1, int opt, int rsz) {
+    for (int o = 0; o < opt; ++o)
+        if (epw_of(n1, o, rsz) == epw_of(n1, opt, rsz) && matrix_volume(n1, o, rsz) == matrix_volume(n1, opt, rsz))
+            return o;
+    return opt;
 }
 
 // Workgroups resident per CU for one kernel instance (both FIRST variants
@@ -894,7 +904,8 @@ using ResidentFn = int (*)();
 
 template <int N1, typename R, typename KR>
 ResidentFn pick_resident(int option) {
-    return option == 0 ? &resident_of<N1, 0, R, KR> : &resident_of<N1, 1, R, KR>;
+    constexpr int O2 = canonical_option(N1, 2, sizeof(R));
+    return option == 1 ? &resident_of<N1, 1, R, KR> : (option == 2 ? &resident_of<N1, O2, R, KR> : &resident_of<N1, 0, R, KR>);
 }
 
 template <typename R, typename KR>
@@ -914,7 +925,8 @@ ResidentFn pick_resident_n1(int n1, int option) {
 
 template <int N1, typename R, typename KR>
 LaunchFn pick(int option) {
-    return option == 0 ? &launch_one<N1, 0, R, KR> : &launch_one<N1, 1, R, KR>;
+    constexpr int O2 = canonical_option(N1, 2, sizeof(R));
+    return option == 1 ? &launch_one<N1, 1, R, KR> : (option == 2 ? &launch_one<N1, O2, R, KR> : &launch_one<N1, 0, R, KR>);
 }
 
 template <typename R, typename KR>
@@ -934,14 +946,21 @@ LaunchFn pick_n1(int n1, int option) {
 
 }  // namespace
 
+static int clamp_option(int option) { return option < 0 ? 0 : (option >= kNumTileOptions ? kNumTileOptions - 1 : option); }
+
 int fused_epw(int n1, int option, int prec) {
     if (n1 < 0 || n1 > kMaxN1) return 0;
     if (prec == kBF16X3) return 1;
-    return epw_of(n1, option ? 1 : 0, prec == kFP64 ? 8 : 4);
+    return epw_of(n1, clamp_option(option), prec == kFP64 ? 8 : 4);
 }
 
 bool fused_matrix_volume(int n1, int option, int prec) {
-    return prec == kBF16X3 || matrix_volume(n1, option ? 1 : 0, prec == kFP64 ? 8 : 4);
+    return prec == kBF16X3 || matrix_volume(n1, clamp_option(option), prec == kFP64 ? 8 : 4);
+}
+
+int fused_option_of(int n1, int option, int prec) {
+    if (n1 < 0 || n1 > kMaxN1 || prec == kBF16X3) return 0;
+    return canonical_option(n1, clamp_option(option), prec == kFP64 ? 8 : 4);
 }
 
 bool fused_supported(int n1, int prec) {

@@ -41,8 +41,13 @@ struct FusedArgs {
     int affine;               // every element affine: records hold the geometry (launch_affine_records)
 };
 
-// Elements per workgroup options (index 0/1) for a given n1.
+// Tile options per n1: elements per workgroup (and at n1 = 8 in FP32, option 1,
+// the matrix-core volume).
+constexpr int kNumTileOptions = 3;
 int fused_epw(int n1, int option, int prec);
+// The lowest option that builds the same kernel as `option` (options that
+// repeat a shape are not timed twice by the tuner).
+int fused_option_of(int n1, int option, int prec);
 // Workgroups of that shape resident on the current device at once (0: not built).
 int fused_resident(int n1, int prec, int option);
 // Whether tile option `option` computes the volume contractions on the matrix cores.

@@ -267,7 +267,7 @@ int Solver::init(const Mesh* const* meshes, int count, double c0_, const nw_opts
     pipe = o.pipeline;
     use_graph = o.use_graph != 0;
     autotune = (o.tile_option < 0) && pipe == NW_PIPE_FUSED;
-    const int opt = o.tile_option < 0 ? 0 : (o.tile_option ? 1 : 0);
+    const int opt = o.tile_option < 0 ? 0 : std::min(o.tile_option, kNumTileOptions - 1);
     c0 = c0_;
     NW_TRY_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     NW_TRY_HIP(hipEventCreate(&ev0));
@@ -386,7 +386,7 @@ int Solver::init(const Mesh* const* meshes, int count, double c0_, const nw_opts
         }
         if ((st = upload_vec(&d_fnb, fnb))) return st;
         std::vector<int> gorder, gseq;
-        for (int opt = 0; opt < 2; ++opt)
+        for (int opt = 0; opt < kNumTileOptions; ++opt)
             for (Segment& sg : seg) {
                 const int epw = fused_epw(sg.n1, opt, prec);
                 const int resident = fused_resident(sg.n1, prec, opt);
@@ -549,9 +549,9 @@ int Solver::launch_segment(const Segment& sg, int s, int parity, int option, hip
     a.mat = d_mat;
     a.gmat = d_gmat;
     a.fnb = d_fnb + size_t(sg.elem_base) * 6;
-    a.gorder = d_gorder + sg.goff[option ? 1 : 0];
-    a.gseq = d_gseq + sg.soff[option ? 1 : 0];
-    a.nseq = sg.nseq[option ? 1 : 0];
+    a.gorder = d_gorder + sg.goff[option];
+    a.gseq = d_gseq + sg.soff[option];
+    a.nseq = sg.nseq[option];
     a.dt = d_dt;
     a.a = kRkA[s];
     a.b = kRkB[s];
@@ -791,11 +791,12 @@ std::mutex g_tune_mu;
 std::map<TuneKey, int> g_tune;
 }  // namespace
 
-// Autotuner: per n1 segment, time both tile options on a regular stage (it
-// reads k and writes u_out and k, all rewritten by the first real stage, which
-// does not read k) and keep the faster.  The options alternate twice and each
-// keeps its best time, so a slow moment of the GPU does not decide.  Runs at a
-// step boundary, before graphs are captured.
+// Autotuner: per n1 segment, time every distinct tile option (elements per
+// workgroup, or at n1 = 8 the matrix-core volume) on a regular stage (it reads k
+// and writes u_out and k, all rewritten by the first real stage, which does not
+// read k) and keep the fastest.  The options alternate twice and each keeps its
+// best time, so a slow moment of the GPU does not decide.  Runs at a step
+// boundary, before graphs are captured.
 int Solver::tune() {
     if (pipe != NW_PIPE_FUSED) return NW_OK;
     NW_TRY_HIP(hipSetDevice(device));
@@ -809,13 +810,16 @@ int Solver::tune() {
                 continue;
             }
         }
-        float best[2] = {1e30f, 1e30f};
-        for (int opt = 0; opt < 2; ++opt) {  // warm both (first launch sets up the kernel)
+        float best[kNumTileOptions];
+        for (int opt = 0; opt < kNumTileOptions; ++opt) {  // warm each (first launch sets up the kernel)
+            best[opt] = 1e30f;
+            if (fused_option_of(sg.n1, opt, prec) != opt) continue;  // the same kernel as an earlier option
             const int st = launch_segment(sg, 1, cur, opt, stream);
             if (st) return st;
         }
         for (int rep = 0; rep < 2; ++rep)
-            for (int opt = 0; opt < 2; ++opt) {
+            for (int opt = 0; opt < kNumTileOptions; ++opt) {
+                if (fused_option_of(sg.n1, opt, prec) != opt) continue;
                 NW_TRY_HIP(hipEventRecord(ev0, stream));
                 for (int r = 0; r < 3; ++r) {
                     const int st = launch_segment(sg, 1, cur, opt, stream);
@@ -827,7 +831,9 @@ int Solver::tune() {
                 NW_TRY_HIP(hipEventElapsedTime(&ms, ev0, ev1));
                 best[opt] = std::min(best[opt], ms);
             }
-        sg.option = best[1] < best[0] ? 1 : 0;
+        sg.option = 0;
+        for (int opt = 1; opt < kNumTileOptions; ++opt)
+            if (best[opt] < best[sg.option]) sg.option = opt;
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tune[key] = sg.option;
     }

@@ -4,6 +4,7 @@
 
 #include <vector>
 
+#include "nw_fused.hpp"
 #include "nw_mesh.hpp"
 
 namespace nw {
@@ -20,9 +21,9 @@ struct Segment {
     int n1 = 0, nelt = 0, elem_base = 0, option = 0;
     bool affine = false;  // every element a parallelepiped: records hold the geometry
     long long node_base = 0;
-    long long goff[2] = {0, 0};  // offset of this segment's group order, per tile option
-    long long soff[2] = {0, 0};  // offset of its workgroup sequence offsets, per tile option
-    int nseq[2] = {0, 0};        // its workgroup sequences, per tile option
+    long long goff[kNumTileOptions] = {};  // offset of this segment's group order, per tile option
+    long long soff[kNumTileOptions] = {};  // offset of its workgroup sequence offsets, per tile option
+    int nseq[kNumTileOptions] = {};        // its workgroup sequences, per tile option
 };
 
 // FNV-1a hash of what makes a mesh this mesh: n1, corner coordinates (bit

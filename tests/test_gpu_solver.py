@@ -167,7 +167,8 @@ def test_graph_replays_with_host_work_between():
     s.finalize()
 
 
-@pytest.mark.parametrize("name", ["media_pec_p5", "periodic_jitter_p7", "media_pec_p7"])
+@pytest.mark.parametrize("name", ["media_pec_p5", "periodic_jitter_p7", "media_pec_p7", "halo_media_p4",
+                                  "periodic_jitter_p3"])
 def test_tile_options_agree(name):
     """Both tile options give the same bits: other elements per workgroup, and
     at n1 = 8 in FP32 the matrix-core volume (one fused multiply-add per step,
@@ -177,8 +178,9 @@ def test_tile_options_agree(name):
     dt = o.auto_dt()
     for prec in ("fp64", "fp32", "fp32_k64"):
         a = run_product(m, st0, dt, 3, c0, prec, "fused", tile_option=0)
-        b = run_product(m, st0, dt, 3, c0, prec, "fused", tile_option=1)
-        assert np.array_equal(a, b), prec
+        for opt in (1, 2):
+            b = run_product(m, st0, dt, 3, c0, prec, "fused", tile_option=opt)
+            assert np.array_equal(a, b), (prec, opt)
 
 
 def test_segment_option_reports_matrix_volume():
