@@ -56,12 +56,27 @@ def make_solver(nw, wl, precision, pipeline="fused"):
     return s
 
 
+SETTLE_S = 0.3
+
+
+def settle(s, dt, seconds=SETTLE_S):
+    """Untimed steps for about `seconds` of device time, queued just before a
+    timing, so that the timed region starts with the GPU at its sustained
+    clocks: after host-side set-up the clocks ramp for tens of milliseconds, and
+    a timing of a few milliseconds that starts right away measures the ramp (five
+    back-to-back timings of 20 steps of the 16^3 mesh fell 12 % from first to
+    last)."""
+    per = s.time(dt, 10) / 10
+    s.run(dt, max(1, int(seconds * 1e3 / per)))
+
+
 def time_path(nw, wl, precision, pipeline, steps, warmup, repeats=5):
-    """`warmup` untimed steps, then `repeats` timings of exactly `steps` steps,
-    queued back to back on the solver's stream (HIP events around each; no host
-    upload or wait between the warm-up and the timings).  Returns the per-repeat
-    milliseconds."""
+    """A clock-settling run (`settle`), `warmup` untimed steps, then `repeats`
+    timings of exactly `steps` steps, queued back to back on the solver's stream
+    (HIP events around each; no host upload or wait between the warm-up and the
+    timings).  Returns the per-repeat milliseconds."""
     s = make_solver(nw, wl, precision, pipeline)
+    settle(s, wl.dt)
     t0 = time.perf_counter()
     ms = s.timeRepeats(wl.dt, steps, repeats, warmup=warmup)
     wall = time.perf_counter() - t0
@@ -161,24 +176,31 @@ def load_traffic(name):
     return None
 
 
-def profile_run(nw, W, precision, steps, workload="c2"):
-    """A few steps without graphs, so a kernel trace sees every launch: tile
-    option 0 for a single mesh; for the batched cavities the tuned option of each
-    n1 segment (one persistent launch per segment and stage)."""
+def profile_run(nw, W, precision, steps, workload="c2", option=-1):
+    """A few steps without graphs, so a kernel trace sees every launch.  A single
+    mesh runs tile option `option` (the one the autotuner picks in the timed
+    benchmark, given explicitly so that every counter pass profiles the same
+    kernel: under a counter pass the tuner's own timings are not meaningful);
+    with option -1, and for the batched cavities, the autotuner picks per n1
+    segment (its timing launches of every option come first; the batch runs one
+    persistent launch per segment and stage)."""
     import ctypes
 
     wl = getattr(W, workload)()
     if len(wl.meshes) == 1:
-        s = nw.DgSolver(precision=precision, pipeline="fused", use_graph=False, tile_option=0)
+        kw = {} if option < 0 else {"tile_option": option}
+        s = nw.DgSolver(precision=precision, pipeline="fused", use_graph=False, **kw)
         s.initialize(wl.meshes[0], wl.c0)
         s.uploadState(wl.states[0])
+        if option < 0:
+            s.tune()
     else:
         s = nw.BatchSolver(precision=precision, use_graph=False)
         s.initialize(wl.meshes, wl.c0)
         for x, st in enumerate(wl.states):
             s.uploadState(x, st)
         s.tune()
-        print("segments:", s.segments(), flush=True)
+    print("segments:", s.segments(), flush=True)
     s.run(wl.dt, steps)
     s.finalize()
     ms = ctypes.c_double()
@@ -331,6 +353,8 @@ def main():
     ap.add_argument("--configs", default="c3,c4,c5", help="other workloads to time (comma list, or 'none')")
     ap.add_argument("--profile-workload", default="c2", choices=["c2", "c3", "c4", "c5"],
                     help="workload of --profile-run (single-mesh workloads)")
+    ap.add_argument("--profile-option", type=int, default=-1,
+                    help="tile option of --profile-run on a single mesh (-1: the autotuner's pick)")
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for rocprofv3 counter passes: the fused stage on the workload "
                          "plus a 256 MiB device copy as the HBM byte-count calibration")
@@ -344,7 +368,8 @@ def main():
     from nekwave_amd import workloads as W
 
     if args.profile_run:
-        profile_run(nw, W, args.precision, steps=min(args.steps, 4), workload=args.profile_workload)
+        profile_run(nw, W, args.precision, steps=min(args.steps, 4), workload=args.profile_workload,
+                    option=args.profile_option)
         return
 
     dev = nw.device_info(0)
@@ -412,8 +437,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms / args.steps,
-        "timing": "median of 5 back-to-back timings of exactly `steps` steps after `warmup` untimed steps "
-                  "(HIP events on the solver's stream)",
+        "timing": "median of 5 back-to-back timings of exactly `steps` steps after an untimed %.1f s "
+                  "clock-settling run and `warmup` untimed steps (HIP events on the solver's stream)" % SETTLE_S,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
