@@ -127,15 +127,20 @@ typedef __attribute__((ext_vector_type(2))) double f64x2;
 // (Each 8-byte word is reinterpreted whole as two floats: extracting single
 // 32-bit lanes of the loaded vector before the reinterpretation let the
 // compiler narrow the access to its first lane.)
+// CP: the access's cache-policy bits (kNT: non-temporal -- no reuse expected,
+// so the line is a first candidate for eviction from L2).
+constexpr int kNT = 2;
+template <int CP = 0>
 __device__ __forceinline__ void ld6(__amdgpu_buffer_rsrc_t r, unsigned vo, float (&o)[6]) {
-    const f32x4 a = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, vo, 0, 0));
-    const f32x2 b = __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(r, vo + 16u, 0, 0));
+    const f32x4 a = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, vo, 0, CP));
+    const f32x2 b = __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(r, vo + 16u, 0, CP));
     o[0] = a[0]; o[1] = a[1]; o[2] = a[2]; o[3] = a[3]; o[4] = b[0]; o[5] = b[1];
 }
+template <int CP = 0>
 __device__ __forceinline__ void ld6(__amdgpu_buffer_rsrc_t r, unsigned vo, double (&o)[6]) {
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
-        const f64x2 a = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(r, vo + 16u * x, 0, 0));
+        const f64x2 a = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(r, vo + 16u * x, 0, CP));
         o[2 * x] = a[0];
         o[2 * x + 1] = a[1];
     }
@@ -153,17 +158,48 @@ __device__ __forceinline__ void ld4(__amdgpu_buffer_rsrc_t r, unsigned vo, doubl
         o[2 * x + 1] = a[1];
     }
 }
+// The LSRK register k of one node.  k is element-local (only the node's own
+// thread reads and writes it), so its layout serves the stores: per element six
+// component blocks of NP values, element e's block c at node index
+// 6 e0 + c NP (e0 = the element's first node), so a wave's 64 lanes write (and
+// read) whole cache lines per component instead of 24-byte strides.
+template <int CP, int NP>
+__device__ __forceinline__ void ldk(__amdgpu_buffer_rsrc_t r, unsigned e0, unsigned m, float (&o)[6]) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+        o[c] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (e0 * 6u + unsigned(c * NP) + m) * 4u, 0, CP));
+}
+template <int CP, int NP>
+__device__ __forceinline__ void ldk(__amdgpu_buffer_rsrc_t r, unsigned e0, unsigned m, double (&o)[6]) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+        o[c] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (e0 * 6u + unsigned(c * NP) + m) * 8u, 0, CP));
+}
+template <int CP, int NP>
+__device__ __forceinline__ void stk(__amdgpu_buffer_rsrc_t r, unsigned e0, unsigned m, const float (&v)[6]) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v[c]), r, (e0 * 6u + unsigned(c * NP) + m) * 4u, 0, CP);
+}
+template <int CP, int NP>
+__device__ __forceinline__ void stk(__amdgpu_buffer_rsrc_t r, unsigned e0, unsigned m, const double (&v)[6]) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v[c]), r, (e0 * 6u + unsigned(c * NP) + m) * 8u, 0, CP);
+}
+template <int CP = 0>
 __device__ __forceinline__ void st6(__amdgpu_buffer_rsrc_t r, unsigned vo, const float (&v)[6]) {
     const f32x4 a = {v[0], v[1], v[2], v[3]};
     const f32x2 b = {v[4], v[5]};
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, a), r, vo, 0, 0);
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, b), r, vo + 16u, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, a), r, vo, 0, CP);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, b), r, vo + 16u, 0, CP);
 }
+template <int CP = 0>
 __device__ __forceinline__ void st6(__amdgpu_buffer_rsrc_t r, unsigned vo, const double (&v)[6]) {
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
         const f64x2 a = {v[2 * x], v[2 * x + 1]};
-        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, a), r, vo + 16u * x, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, a), r, vo + 16u * x, 0, CP);
     }
 }
 
@@ -320,7 +356,7 @@ void k_fused_stage(FusedArgs A) {
     const __amdgpu_buffer_rsrc_t rU = make_rsrc(A.uin), rO = make_rsrc(A.uout), rK = make_rsrc(A.kbuf);
     const __amdgpu_buffer_rsrc_t rW = make_rsrc(A.fw), rN = make_rsrc(A.fnb);
     const __amdgpu_buffer_rsrc_t rR = make_rsrc(A.rec);
-    constexpr unsigned UB = 6 * sizeof(R), KB = 6 * sizeof(KR);  // bytes per node of u and of k
+    constexpr unsigned UB = 6 * sizeof(R);  // bytes per node of u
 
     // loop-invariant: node coordinates, inverse mass weight
     const double zr = c_gll.z[N1][i], zs = c_gll.z[N1][j], zt = c_gll.z[N1][k];
@@ -498,7 +534,13 @@ void k_fused_stage(FusedArgs A) {
         // before the prefetch: waiting on them does not wait on it
         const unsigned nd = node_of(g);
         KR kold[6] = {KR(0), KR(0), KR(0), KR(0), KR(0), KR(0)};
-        if (!FIRST) ld6(rK, nd * KB, kold);
+        const unsigned e0 = nd - unsigned(m);  // the element's first node
+        if (!FIRST) {  // non-temporal where the launcher asks (stream_k: no L2 reuse, whole lines)
+            if (A.stream_k)
+                ldk<kNT, NP>(rK, e0, unsigned(m), kold);
+            else
+                ldk<0, NP>(rK, e0, unsigned(m), kold);
+        }
         R fw[FPT][4];
 #pragma unroll
         for (int t = 0; t < FPT; ++t) {
@@ -804,7 +846,10 @@ void k_fused_stage(FusedArgs A) {
                 kn[c] = FIRST ? KR(dt) * KR(rhs) : KR(A.a) * kold[c] + KR(dt) * KR(rhs);
                 un[c] = R(KR(sU[es][c][m]) + KR(A.b) * kn[c]);
             }
-            st6(rK, nd * KB, kn);
+            if (A.stream_k)
+                stk<kNT, NP>(rK, e0, unsigned(m), kn);
+            else
+                stk<0, NP>(rK, e0, unsigned(m), kn);
             st6(rO, nd * UB, un);
         }
         geometry(sE[(t + 1) & 1][es], geo);  // the next group's record, staged in this iteration
@@ -961,6 +1006,10 @@ bool fused_matrix_volume(int n1, int option, int prec) {
 int fused_option_of(int n1, int option, int prec) {
     if (n1 < 0 || n1 > kMaxN1 || prec == kBF16X3) return 0;
     return canonical_option(n1, clamp_option(option), prec == kFP64 ? 8 : 4);
+}
+
+bool fused_stream_k(int n1, long long ws_bytes) {
+    return (n1 * n1 * n1) % 64 == 0 && ws_bytes > (256ll << 20);
 }
 
 bool fused_supported(int n1, int prec) {

@@ -13,9 +13,9 @@ enum Prec { kFP64 = 0, kFP32 = 1, kFP32K64 = 2, kBF16X3 = 3, kNumPrec = 4 };
 // contiguous range of elements that share n1 (a whole mesh, or one n1 segment
 // of a batched set of independent meshes).
 struct FusedArgs {
-    const void* uin;          // 6 component blocks of R, block stride `stride`
+    const void* uin;          // node-major: the 6 values (R) of each node consecutive; ghosts after npts
     void* uout;               // same layout
-    void* kbuf;               // 6 component blocks of KR, block stride `kstride`
+    void* kbuf;               // LSRK register (KR): per element 6 component blocks of n1^3 values
     long long stride;         // npts_total + nghost (ghost slots after the local nodes)
     long long kstride;
     long long node_base;      // first node of this segment
@@ -39,7 +39,14 @@ struct FusedArgs {
     const void* fw;           // per element face (nelt*6, segment-local), 4 R: flux weights
                               // (wEh, wEe, wHe, wHh), see face_flux_weights
     int affine;               // every element affine: records hold the geometry (launch_affine_records)
+    int stream_k;             // k accesses non-temporal (set by the caller: fused_stream_k)
 };
+
+// Whether the LSRK register accesses of a segment should be non-temporal: only
+// where every wave reads and writes whole cache lines of k (n1^3 a multiple of
+// 64) and the stage's working set (u in, u out, k: `ws_bytes`) is larger than
+// the 256 MiB Infinity Cache, which a non-temporal stream would bypass.
+bool fused_stream_k(int n1, long long ws_bytes);
 
 // Tile options per n1: elements per workgroup (and at n1 = 8 in FP32, option 1,
 // the matrix-core volume).

@@ -558,6 +558,11 @@ int Solver::launch_segment(const Segment& sg, int s, int parity, int option, hip
     a.c0 = c0;
     a.first_stage = (s == 0);
     a.affine = sg.affine ? 1 : 0;
+    {
+        const long long su = prec == kFP64 ? 8 : 4;
+        const long long sk = (prec == kFP32 || prec == kBF16X3) ? 4 : 8;
+        a.stream_k = fused_stream_k(sg.n1, npts * 6 * (2 * su + sk)) ? 1 : 0;
+    }
     a.fw = static_cast<const char*>(d_fw) + size_t(sg.elem_base) * 24 * (prec == kFP64 ? 8 : 4);
     NW_TRY_HIP(launch_fused(sg.n1, prec, option, a, st));
     return NW_OK;
