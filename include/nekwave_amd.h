@@ -207,15 +207,27 @@ int nw_dg_probe_history(nw_dg* s, double* out, int max_records, int* nrecords);
 /* Host half of ProbeManager::init (probe.cpp:57-96): element and clamped (r,s,t)
  * of each probe. */
 int nw_mesh_locate_probes(const nw_mesh* m, int n, const double* xyz, int* elem, double* rst);
-/* Autotuner: time every distinct tile option per segment and keep the fastest
- * (runs by itself before the first step when tile_option = -1). */
+/* Autotuner: time every distinct tile option per segment -- with and without
+ * non-temporal k accesses where the working set exceeds the 256 MiB Infinity
+ * Cache -- and keep the fastest (runs by itself before the first step when
+ * tile_option = -1).  Choices are kept per process, keyed by device, n1,
+ * precision, element count and single mesh or batch. */
 int nw_dg_tune(nw_dg* s);
 /* n1, element count and chosen elements per workgroup of segment `seg`. */
 int nw_dg_segment_info(const nw_dg* s, int seg, int* n1, int* nelt, int* epw);
-/* Extension: the tile option segment `seg` runs (0/1/2, as tuned or set) and
+/* Extension: the tile option segment `seg` runs (0/1/2, as tuned or set),
  * whether that option computes the volume contractions on the matrix cores
- * (n1 = 8 with FP32 fields, option 1; results equal the vector path's). */
-int nw_dg_segment_option(const nw_dg* s, int seg, int* option, int* matrix_volume);
+ * (n1 = 8 with FP32 fields, option 1; results equal the vector path's) and
+ * whether its LSRK register accesses are non-temporal (tuned, or by default
+ * where the working set exceeds the 256 MiB Infinity Cache and every wave
+ * covers whole cache lines).  Any output pointer may be NULL. */
+int nw_dg_segment_option(const nw_dg* s, int seg, int* option, int* matrix_volume, int* stream_k);
+/* Extension: set segment `seg`'s tile option (0/1/2) and non-temporal k
+ * accesses (1 on, 0 off, -1 the default rule) -- e.g. choices an earlier
+ * process tuned and kept (nw_dg_segment_option).  The solver then counts as
+ * tuned: autotuning does not override the choice until nw_dg_tune is called.
+ * Results do not depend on either choice. */
+int nw_dg_set_segment_option(nw_dg* s, int seg, int option, int stream_k);
 /* DgSolver::finalize (dg_solver.cpp:529-582) + destructor. */
 void nw_dg_destroy(nw_dg* s);
 

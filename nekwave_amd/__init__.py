@@ -346,20 +346,29 @@ class DgSolver:
         return self._h
 
     def tune(self):
-        """Time both tile options per segment and keep the faster (nw_dg_tune)."""
+        """Time every distinct tile option per segment (with and without
+        non-temporal k accesses where the working set exceeds the Infinity
+        Cache) and keep the fastest (nw_dg_tune)."""
         check(lib().nw_dg_tune(self._handle()))
 
+    def setSegmentOption(self, seg, option, stream_k=-1):
+        """Set segment `seg`'s tile option and non-temporal k accesses (1, 0, or
+        -1 for the default rule), e.g. as an earlier process tuned them
+        (`segments()`); autotuning then keeps them until `tune()`."""
+        check(lib().nw_dg_set_segment_option(self._handle(), int(seg), int(option), int(stream_k)))
+
     def segments(self):
-        """Per n1 segment: n1, elements, elements per workgroup, tile option and
-        whether it runs the volume contractions on the matrix cores."""
+        """Per n1 segment: n1, elements, elements per workgroup, tile option,
+        whether it runs the volume contractions on the matrix cores and whether
+        its LSRK register accesses are non-temporal."""
         out = []
         for s in range(self.info()["segments"]):
             n1, nelt, epw = C.c_int(), C.c_int(), C.c_int()
             check(lib().nw_dg_segment_info(self._handle(), s, C.byref(n1), C.byref(nelt), C.byref(epw)))
-            opt, mx = C.c_int(), C.c_int()
-            check(lib().nw_dg_segment_option(self._handle(), s, C.byref(opt), C.byref(mx)))
+            opt, mx, sk = C.c_int(), C.c_int(), C.c_int()
+            check(lib().nw_dg_segment_option(self._handle(), s, C.byref(opt), C.byref(mx), C.byref(sk)))
             out.append({"n1": n1.value, "nelt": nelt.value, "epw": epw.value, "option": opt.value,
-                        "matrix_volume": bool(mx.value)})
+                        "matrix_volume": bool(mx.value), "stream_k": bool(sk.value)})
         return out
 
     def __del__(self):

@@ -298,6 +298,10 @@ NW_API int nw_dg_tune(nw_dg* s) {
     }
     return s->s.tune();
 }
+NW_API int nw_dg_set_segment_option(nw_dg* s, int seg, int option, int stream_k) {
+    NW_CHECK_ARG(s, "null solver");
+    return s->s.set_segment_option(seg, option, stream_k);
+}
 NW_API int nw_dg_segment_info(const nw_dg* s, int seg, int* n1, int* nelt, int* epw) {
     NW_CHECK_ARG(s && seg >= 0 && seg < (int)s->s.seg.size(), "bad segment");
     const nw::Segment& g = s->s.seg[seg];
@@ -306,12 +310,13 @@ NW_API int nw_dg_segment_info(const nw_dg* s, int seg, int* n1, int* nelt, int* 
     if (epw) *epw = nw::fused_epw(g.n1, g.option, s->s.prec);
     return NW_OK;
 }
-NW_API int nw_dg_segment_option(const nw_dg* s, int seg, int* option, int* matrix_volume) {
+NW_API int nw_dg_segment_option(const nw_dg* s, int seg, int* option, int* matrix_volume, int* stream_k) {
     NW_CHECK_ARG(s && seg >= 0 && seg < (int)s->s.seg.size(), "bad segment");
     const nw::Segment& g = s->s.seg[seg];
     const bool fused = s->s.pipe == NW_PIPE_FUSED;
     if (option) *option = g.option;
     if (matrix_volume) *matrix_volume = fused && nw::fused_matrix_volume(g.n1, g.option, s->s.prec) ? 1 : 0;
+    if (stream_k) *stream_k = fused ? s->s.stream_of(g, g.option) : 0;
     return NW_OK;
 }
 NW_API int nw_dg_step(nw_dg* s, double dt, double time) {

@@ -158,34 +158,36 @@ __device__ __forceinline__ void ld4(__amdgpu_buffer_rsrc_t r, unsigned vo, doubl
         o[2 * x + 1] = a[1];
     }
 }
-// The LSRK register k of one node.  k is element-local (only the node's own
-// thread reads and writes it), so its layout serves the stores: per element six
-// component blocks of NP values, element e's block c at node index
-// 6 e0 + c NP (e0 = the element's first node), so a wave's 64 lanes write (and
-// read) whole cache lines per component instead of 24-byte strides.
-template <int CP, int NP>
-__device__ __forceinline__ void ldk(__amdgpu_buffer_rsrc_t r, unsigned e0, unsigned m, float (&o)[6]) {
+// The LSRK register k of one node.  k is private to the thread that updates
+// the node (it lives only within one step), so its layout serves the accesses:
+// per wave of a group, six component blocks of 64 values (a chunk), component c
+// of lane l at chunk offset 64 c + l -- every wave reads and writes whole cache
+// lines of each component, whatever the shape of the group.  vo: the lane's
+// byte offset in its wave's chunk (loop-invariant); so: the chunk's byte offset
+// (wave-uniform, in the instruction's scalar offset).
+template <int CP>
+__device__ __forceinline__ void ldk(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so, float (&o)[6]) {
 #pragma unroll
     for (int c = 0; c < 6; ++c)
-        o[c] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (e0 * 6u + unsigned(c * NP) + m) * 4u, 0, CP));
+        o[c] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo + 256u * c, so, CP));
 }
-template <int CP, int NP>
-__device__ __forceinline__ void ldk(__amdgpu_buffer_rsrc_t r, unsigned e0, unsigned m, double (&o)[6]) {
+template <int CP>
+__device__ __forceinline__ void ldk(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so, double (&o)[6]) {
 #pragma unroll
     for (int c = 0; c < 6; ++c)
-        o[c] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (e0 * 6u + unsigned(c * NP) + m) * 8u, 0, CP));
+        o[c] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo + 512u * c, so, CP));
 }
-template <int CP, int NP>
-__device__ __forceinline__ void stk(__amdgpu_buffer_rsrc_t r, unsigned e0, unsigned m, const float (&v)[6]) {
+template <int CP>
+__device__ __forceinline__ void stk(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so, const float (&v)[6]) {
 #pragma unroll
     for (int c = 0; c < 6; ++c)
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v[c]), r, (e0 * 6u + unsigned(c * NP) + m) * 4u, 0, CP);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v[c]), r, vo + 256u * c, so, CP);
 }
-template <int CP, int NP>
-__device__ __forceinline__ void stk(__amdgpu_buffer_rsrc_t r, unsigned e0, unsigned m, const double (&v)[6]) {
+template <int CP>
+__device__ __forceinline__ void stk(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so, const double (&v)[6]) {
 #pragma unroll
     for (int c = 0; c < 6; ++c)
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v[c]), r, (e0 * 6u + unsigned(c * NP) + m) * 8u, 0, CP);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v[c]), r, vo + 512u * c, so, CP);
 }
 template <int CP = 0>
 __device__ __forceinline__ void st6(__amdgpu_buffer_rsrc_t r, unsigned vo, const float (&v)[6]) {
@@ -295,6 +297,7 @@ __global__ __launch_bounds__(EPW* N1* N1* N1) __attribute__((amdgpu_waves_per_eu
 void k_fused_stage(FusedArgs A) {
     constexpr int N2 = N1 * N1, NP = N1 * N2, NF = 6 * N2, NT = EPW * NP;
     constexpr int FPT = (EPW * NF + NT - 1) / NT;  // face points per thread (1 for n1 >= 6)
+    constexpr int NW = (NT + 63) / 64;             // waves per workgroup (k chunks per group)
     constexpr int ERW = 26;                         // doubles per element record: geometry (24), 1/eps, 1/mu
     constexpr int GQ = (EPW * ERW + NT - 1) / NT;   // element-record words per thread
     struct ElemRec {
@@ -357,6 +360,7 @@ void k_fused_stage(FusedArgs A) {
     const __amdgpu_buffer_rsrc_t rW = make_rsrc(A.fw), rN = make_rsrc(A.fnb);
     const __amdgpu_buffer_rsrc_t rR = make_rsrc(A.rec);
     constexpr unsigned UB = 6 * sizeof(R);  // bytes per node of u
+    const unsigned kv = unsigned(384 * (tid >> 6) + (tid & 63)) * unsigned(sizeof(KR));  // lane's k offset in the group's chunks
 
     // loop-invariant: node coordinates, inverse mass weight
     const double zr = c_gll.z[N1][i], zs = c_gll.z[N1][j], zt = c_gll.z[N1][k];
@@ -534,12 +538,13 @@ void k_fused_stage(FusedArgs A) {
         // before the prefetch: waiting on them does not wait on it
         const unsigned nd = node_of(g);
         KR kold[6] = {KR(0), KR(0), KR(0), KR(0), KR(0), KR(0)};
-        const unsigned e0 = nd - unsigned(m);  // the element's first node
-        if (!FIRST) {  // non-temporal where the launcher asks (stream_k: no L2 reuse, whole lines)
+        // group g's waves hold k chunks kbase + NW g + (0 .. NW-1)
+        const unsigned ks = (unsigned(A.kbase) + unsigned(g) * unsigned(NW)) * (384u * sizeof(KR));
+        if (!FIRST) {  // non-temporal where the caller asks (stream_k: no L2 reuse)
             if (A.stream_k)
-                ldk<kNT, NP>(rK, e0, unsigned(m), kold);
+                ldk<kNT>(rK, kv, ks, kold);
             else
-                ldk<0, NP>(rK, e0, unsigned(m), kold);
+                ldk<0>(rK, kv, ks, kold);
         }
         R fw[FPT][4];
 #pragma unroll
@@ -847,9 +852,9 @@ void k_fused_stage(FusedArgs A) {
                 un[c] = R(KR(sU[es][c][m]) + KR(A.b) * kn[c]);
             }
             if (A.stream_k)
-                stk<kNT, NP>(rK, e0, unsigned(m), kn);
+                stk<kNT>(rK, kv, ks, kn);
             else
-                stk<0, NP>(rK, e0, unsigned(m), kn);
+                stk<0>(rK, kv, ks, kn);
             st6(rO, nd * UB, un);
         }
         geometry(sE[(t + 1) & 1][es], geo);  // the next group's record, staged in this iteration
@@ -926,6 +931,9 @@ hipError_t launch_one(const FusedArgs& a, hipStream_t s) {
     if (span * sizeof(double) >= (1ull << 32)) return hipErrorInvalidValue;
     FusedArgs b = a;
     b.ngroups = (a.nelt + EPW - 1) / EPW;
+    // the segment's k chunks (one per wave and group) lie inside the k buffer
+    if (a.kbase < 0 || (a.kbase + (long long)b.ngroups * fused_k_chunks_per_group(NT)) * 64 > a.kstride)
+        return hipErrorInvalidValue;
     if (!a.gorder || !a.gseq || a.nseq <= 0 || a.nseq > b.ngroups) return hipErrorInvalidValue;
     b.wchunk = (a.nseq + 7) / 8;
     b.gstride = 8 * b.wchunk;  // workgroups past nseq return at once
@@ -1008,8 +1016,14 @@ int fused_option_of(int n1, int option, int prec) {
     return canonical_option(n1, clamp_option(option), prec == kFP64 ? 8 : 4);
 }
 
-bool fused_stream_k(int n1, long long ws_bytes) {
-    return (n1 * n1 * n1) % 64 == 0 && ws_bytes > (256ll << 20);
+bool fused_stream_k(long long ws_bytes, int threads, int ksize) {
+    return ws_bytes > (256ll << 20) && ((threads % 64) * ksize) % 128 == 0;
+}
+
+long long fused_k_chunks(int n1, int option, int prec, int nelt) {
+    const int epw = fused_epw(n1, option, prec);
+    if (epw <= 0) return 0;
+    return (long long)((nelt + epw - 1) / epw) * fused_k_chunks_per_group(epw * n1 * n1 * n1);
 }
 
 bool fused_supported(int n1, int prec) {

@@ -15,9 +15,10 @@ enum Prec { kFP64 = 0, kFP32 = 1, kFP32K64 = 2, kBF16X3 = 3, kNumPrec = 4 };
 struct FusedArgs {
     const void* uin;          // node-major: the 6 values (R) of each node consecutive; ghosts after npts
     void* uout;               // same layout
-    void* kbuf;               // LSRK register (KR): per element 6 component blocks of n1^3 values
+    void* kbuf;               // LSRK register (KR): per wave of a group, 6 component blocks of 64 lanes
     long long stride;         // npts_total + nghost (ghost slots after the local nodes)
-    long long kstride;
+    long long kstride;        // k values per component in the whole buffer (64 per chunk)
+    long long kbase;          // first k chunk of this segment (fused_k_chunks)
     long long node_base;      // first node of this segment
     long long ghost_base;     // first ghost slot (node index) of this segment
     int nelt;                 // elements in this segment
@@ -42,11 +43,18 @@ struct FusedArgs {
     int stream_k;             // k accesses non-temporal (set by the caller: fused_stream_k)
 };
 
-// Whether the LSRK register accesses of a segment should be non-temporal: only
-// where every wave reads and writes whole cache lines of k (n1^3 a multiple of
-// 64) and the stage's working set (u in, u out, k: `ws_bytes`) is larger than
-// the 256 MiB Infinity Cache, which a non-temporal stream would bypass.
-bool fused_stream_k(int n1, long long ws_bytes);
+// Whether the LSRK register accesses should be non-temporal, by default (the
+// autotuner times both): where the stage's working set (u in, u out, k:
+// `ws_bytes`) is larger than the 256 MiB Infinity Cache, which a non-temporal
+// stream would bypass, and where the last wave of a group of `threads` threads
+// also covers whole cache lines of k (elements `ksize` bytes): a partly written
+// line streamed out is written again when its other part is.
+bool fused_stream_k(long long ws_bytes, int threads, int ksize);
+// k chunks (6 x 64 values each) of one group of `threads` threads, and of a
+// segment of `nelt` elements under a tile option: a step's stages all use one
+// option, so the k buffer holds, per segment, the most any option needs.
+constexpr long long fused_k_chunks_per_group(int threads) { return (threads + 63) / 64; }
+long long fused_k_chunks(int n1, int option, int prec, int nelt);
 
 // Tile options per n1: elements per workgroup (and at n1 = 8 in FP32, option 1,
 // the matrix-core volume).

@@ -371,10 +371,18 @@ int Solver::init(const Mesh* const* meshes, int count, double c0_, const nw_opts
     } else {
         if ((st = dalloc(&d_u[0], rsz * 6 * stride))) return st;
         if ((st = dalloc(&d_u[1], rsz * 6 * stride))) return st;
-        if ((st = dalloc(&d_k, ksz * 6 * npts))) return st;
+        kchunks = 0;
+        for (Segment& sg : seg) {
+            sg.kbase = kchunks;
+            long long most = 0;
+            for (int opt = 0; opt < kNumTileOptions; ++opt)
+                most = std::max(most, fused_k_chunks(sg.n1, opt, prec, sg.nelt));
+            kchunks += most;
+        }
+        if ((st = dalloc(&d_k, ksz * 384 * kchunks))) return st;
         NW_TRY_HIP(hipMemsetAsync(d_u[0], 0, rsz * 6 * stride, stream));
         NW_TRY_HIP(hipMemsetAsync(d_u[1], 0, rsz * 6 * stride, stream));
-        NW_TRY_HIP(hipMemsetAsync(d_k, 0, ksz * 6 * npts, stream));
+        NW_TRY_HIP(hipMemsetAsync(d_k, 0, ksz * 384 * kchunks, stream));
         std::vector<int4> fnb(size_t(nelt) * 6);
         for (int r = 0; r < count; ++r) {
             const Instance& I = inst[r];
@@ -533,13 +541,27 @@ int Solver::set_dt(double dt) {
     return NW_OK;
 }
 
-int Solver::launch_segment(const Segment& sg, int s, int parity, int option, hipStream_t st) {
+long long Solver::working_set_bytes() const {
+    const long long su = prec == kFP64 ? 8 : 4;
+    const long long sk = (prec == kFP32 || prec == kBF16X3) ? 4 : 8;
+    return npts * 6 * (2 * su + sk);
+}
+
+int Solver::stream_of(const Segment& sg, int option) const {
+    if (sg.stream >= 0) return sg.stream;
+    const int threads = fused_epw(sg.n1, option, prec) * sg.n1 * sg.n1 * sg.n1;
+    const int ksize = (prec == kFP32 || prec == kBF16X3) ? 4 : 8;
+    return fused_stream_k(working_set_bytes(), threads, ksize) ? 1 : 0;
+}
+
+int Solver::launch_segment(const Segment& sg, int s, int parity, int option, int stream_k, hipStream_t st) {
     FusedArgs a{};
     a.uin = d_u[parity];
     a.uout = d_u[1 - parity];
     a.kbuf = d_k;
     a.stride = stride;
-    a.kstride = npts;
+    a.kstride = 64 * kchunks;
+    a.kbase = sg.kbase;
     a.node_base = sg.node_base;
     a.ghost_base = npts;
     a.nelt = sg.nelt;
@@ -558,11 +580,7 @@ int Solver::launch_segment(const Segment& sg, int s, int parity, int option, hip
     a.c0 = c0;
     a.first_stage = (s == 0);
     a.affine = sg.affine ? 1 : 0;
-    {
-        const long long su = prec == kFP64 ? 8 : 4;
-        const long long sk = (prec == kFP32 || prec == kBF16X3) ? 4 : 8;
-        a.stream_k = fused_stream_k(sg.n1, npts * 6 * (2 * su + sk)) ? 1 : 0;
-    }
+    a.stream_k = stream_k;
     a.fw = static_cast<const char*>(d_fw) + size_t(sg.elem_base) * 24 * (prec == kFP64 ? 8 : 4);
     NW_TRY_HIP(launch_fused(sg.n1, prec, option, a, st));
     return NW_OK;
@@ -596,7 +614,7 @@ int Solver::enqueue_fused_stage(int s, int parity) {
     // would only compete for the same CUs.
     int st;
     for (size_t x = 0; x < seg.size(); ++x)
-        if ((st = launch_segment(seg[x], s, parity, seg[x].option, stream))) return st;
+        if ((st = launch_segment(seg[x], s, parity, seg[x].option, stream_of(seg[x], seg[x].option), stream))) return st;
     return NW_OK;
 }
 
@@ -793,7 +811,7 @@ struct TuneKey {
     }
 };
 std::mutex g_tune_mu;
-std::map<TuneKey, int> g_tune;
+std::map<TuneKey, std::pair<int, int>> g_tune;  // (tile option, non-temporal k)
 }  // namespace
 
 // Autotuner: per n1 segment, time every distinct tile option (elements per
@@ -805,44 +823,69 @@ std::map<TuneKey, int> g_tune;
 int Solver::tune() {
     if (pipe != NW_PIPE_FUSED) return NW_OK;
     NW_TRY_HIP(hipSetDevice(device));
+    // non-temporal k is a candidate only where the working set exceeds the
+    // Infinity Cache (below it the hint only costs)
+    const int nstream = fused_stream_k(working_set_bytes(), 0, 1) ? 2 : 1;
     for (Segment& sg : seg) {
         const TuneKey key{device, sg.n1, prec, inst.size() == 1 ? 1 : 0, sg.nelt};
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
             auto it = g_tune.find(key);
             if (it != g_tune.end()) {
-                sg.option = it->second;
+                sg.option = it->second.first;
+                sg.stream = it->second.second;
                 continue;
             }
         }
-        float best[kNumTileOptions];
+        float best[kNumTileOptions][2];
         for (int opt = 0; opt < kNumTileOptions; ++opt) {  // warm each (first launch sets up the kernel)
-            best[opt] = 1e30f;
+            best[opt][0] = best[opt][1] = 1e30f;
             if (fused_option_of(sg.n1, opt, prec) != opt) continue;  // the same kernel as an earlier option
-            const int st = launch_segment(sg, 1, cur, opt, stream);
+            const int st = launch_segment(sg, 1, cur, opt, 0, stream);
             if (st) return st;
         }
         for (int rep = 0; rep < 2; ++rep)
-            for (int opt = 0; opt < kNumTileOptions; ++opt) {
-                if (fused_option_of(sg.n1, opt, prec) != opt) continue;
-                NW_TRY_HIP(hipEventRecord(ev0, stream));
-                for (int r = 0; r < 3; ++r) {
-                    const int st = launch_segment(sg, 1, cur, opt, stream);
-                    if (st) return st;
+            for (int opt = 0; opt < kNumTileOptions; ++opt)
+                for (int str = 0; str < nstream; ++str) {
+                    if (fused_option_of(sg.n1, opt, prec) != opt) continue;
+                    NW_TRY_HIP(hipEventRecord(ev0, stream));
+                    for (int r = 0; r < 3; ++r) {
+                        const int st = launch_segment(sg, 1, cur, opt, str, stream);
+                        if (st) return st;
+                    }
+                    NW_TRY_HIP(hipEventRecord(ev1, stream));
+                    NW_TRY_HIP(hipEventSynchronize(ev1));
+                    float ms = 0.0f;
+                    NW_TRY_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+                    best[opt][str] = std::min(best[opt][str], ms);
                 }
-                NW_TRY_HIP(hipEventRecord(ev1, stream));
-                NW_TRY_HIP(hipEventSynchronize(ev1));
-                float ms = 0.0f;
-                NW_TRY_HIP(hipEventElapsedTime(&ms, ev0, ev1));
-                best[opt] = std::min(best[opt], ms);
-            }
         sg.option = 0;
-        for (int opt = 1; opt < kNumTileOptions; ++opt)
-            if (best[opt] < best[sg.option]) sg.option = opt;
+        sg.stream = 0;
+        for (int opt = 0; opt < kNumTileOptions; ++opt)
+            for (int str = 0; str < nstream; ++str)
+                if (best[opt][str] < best[sg.option][sg.stream]) {
+                    sg.option = opt;
+                    sg.stream = str;
+                }
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tune[key] = sg.option;
+        g_tune[key] = {sg.option, sg.stream};
     }
+    drop_graphs();  // captured launches carry the previous choices
     tuned = true;
+    return NW_OK;
+}
+
+int Solver::set_segment_option(int s, int option, int stream_k) {
+    NW_CHECK_ARG(pipe == NW_PIPE_FUSED, "tile options belong to the fused pipeline");
+    NW_CHECK_ARG(s >= 0 && s < static_cast<int>(seg.size()), "bad segment");
+    NW_CHECK_ARG(option >= 0 && option < kNumTileOptions, "tile option must be 0, 1 or 2");
+    NW_CHECK_ARG(stream_k >= -1 && stream_k <= 1, "stream_k must be -1, 0 or 1");
+    NW_TRY_HIP(hipSetDevice(device));
+    NW_TRY_HIP(hipStreamSynchronize(stream));
+    seg[size_t(s)].option = option;
+    seg[size_t(s)].stream = stream_k;
+    tuned = true;  // an explicit choice stands until tune() is called
+    drop_graphs();
     return NW_OK;
 }
 

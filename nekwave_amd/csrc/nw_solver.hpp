@@ -19,8 +19,10 @@ struct Instance {
 // A run of elements sharing n1: one fused launch per stage.
 struct Segment {
     int n1 = 0, nelt = 0, elem_base = 0, option = 0;
+    int stream = -1;  // non-temporal k accesses: 1 / 0 as tuned, -1 the default rule (fused_stream_k)
     bool affine = false;  // every element a parallelepiped: records hold the geometry
     long long node_base = 0;
+    long long kbase = 0;                   // first k chunk (the most any tile option needs, after earlier segments)
     long long goff[kNumTileOptions] = {};  // offset of this segment's group order, per tile option
     long long soff[kNumTileOptions] = {};  // offset of its workgroup sequence offsets, per tile option
     int nseq[kNumTileOptions] = {};        // its workgroup sequences, per tile option
@@ -40,6 +42,7 @@ struct Solver {
     int n1 = 0, np = 0, nelt = 0, kernels_per_step = 0;
     unsigned long long mesh_sig = 0;  // mesh_signature of a single-mesh solver's mesh
     long long npts = 0, tfp = 0, nhalo = 0, stride = 0, device_bytes = 0;
+    long long kchunks = 0;  // fused pipeline: k chunks of 6 x 64 values (Segment::kbase)
     double c0 = 1.0, last_dt = -1.0;
     int cur = 0;
     std::vector<Instance> inst;  // in caller order
@@ -96,6 +99,10 @@ struct Solver {
     int probe_sample();
     int probe_history(double* host, int max_records, int* nrecords);
     int tune();
+    int set_segment_option(int s, int option, int stream_k);
+    // non-temporal k accesses of segment sg under a tile option: the tuned
+    // choice, else the default rule (fused_stream_k)
+    int stream_of(const Segment& sg, int option) const;
 
    private:
     int dalloc(void** p, size_t bytes);
@@ -104,7 +111,8 @@ struct Solver {
     int upload_vec(T** dst, const std::vector<T>& v);
     int init_plain(const Mesh& m);
     int set_dt(double dt);
-    int launch_segment(const Segment& sg, int s, int parity, int option, hipStream_t st);
+    int launch_segment(const Segment& sg, int s, int parity, int option, int stream_k, hipStream_t st);
+    long long working_set_bytes() const;
     int enqueue_fused_stage(int s, int parity);
     int enqueue_plain_stage(int s);
     int enqueue_step(int parity);

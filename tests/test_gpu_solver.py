@@ -183,6 +183,48 @@ def test_tile_options_agree(name):
             assert np.array_equal(a, b), (prec, opt)
 
 
+@pytest.mark.parametrize("name", ["media_pec_p5", "periodic_jitter_p7", "halo_media_p4"])
+def test_segment_choices_give_the_same_bits(name):
+    """Every tile option with the LSRK register accesses non-temporal or not
+    (nw_dg_set_segment_option) gives the bits of tile option 0 without the hint,
+    also when the choice changes between runs of one solver (graphs re-captured),
+    and the solver reports the choice it runs."""
+    m, o, c0 = _build(name)
+    st0 = random_state(o, 3)
+    dt = o.auto_dt()
+    for prec in ("fp32", "fp64"):
+        ref = run_product(m, st0, dt, 4, c0, prec, "fused", tile_option=0)
+        for opt in (0, 1, 2):
+            for sk in (0, 1):
+                s = nw.DgSolver(precision=prec, pipeline="fused", use_graph=True)
+                s.initialize(m, c0)
+                s.uploadState(st0)
+                s.setSegmentOption(0, (opt + 1) % 3, 1 - sk)  # a first choice, replaced after 2 steps
+                s.run(dt, 2)
+                s.setSegmentOption(0, opt, sk)
+                seg = s.segments()[0]
+                assert (seg["option"], seg["stream_k"]) == (opt, bool(sk)), seg
+                s.run(dt, 2)
+                got = s.downloadState()
+                s.finalize()
+                assert np.array_equal(got, ref), (prec, opt, sk)
+
+
+def test_set_segment_option_rejects_bad_arguments():
+    m, o, c0 = _build("media_pec_p5")
+    s = nw.DgSolver(precision="fp32", pipeline="fused")
+    s.initialize(m, c0)
+    for args in [(1, 0, 0), (-1, 0, 0), (0, 3, 0), (0, -1, 0), (0, 0, 2), (0, 0, -2)]:
+        with pytest.raises(nw.NekWaveError):
+            s.setSegmentOption(*args)
+    s.finalize()
+    p = nw.DgSolver(precision="fp64", pipeline="plain")
+    p.initialize(m, c0)
+    with pytest.raises(nw.NekWaveError):
+        p.setSegmentOption(0, 0, 0)
+    p.finalize()
+
+
 def test_segment_option_reports_matrix_volume():
     m, o, c0 = _build("periodic_jitter_p7")
     for opt, prec, want in [(0, "fp32", False), (1, "fp32", True), (1, "fp32_k64", True), (1, "fp64", False)]:
@@ -190,6 +232,8 @@ def test_segment_option_reports_matrix_volume():
         s.initialize(m, c0)
         seg = s.segments()[0]
         s.finalize()
+        # a working set far below the 256 MiB Infinity Cache: k stays temporal
+        assert seg["stream_k"] is False, seg
         assert (seg["n1"], seg["option"], seg["matrix_volume"]) == (8, opt, want), (prec, seg)
 
 
