@@ -176,12 +176,13 @@ def load_traffic(name):
     return None
 
 
-def profile_run(nw, W, precision, steps, workload="c2", option=-1):
+def profile_run(nw, W, precision, steps, workload="c2", option=-1, stream_k=-1):
     """A few steps without graphs, so a kernel trace sees every launch.  A single
-    mesh runs tile option `option` (the one the autotuner picks in the timed
-    benchmark, given explicitly so that every counter pass profiles the same
-    kernel: under a counter pass the tuner's own timings are not meaningful);
-    with option -1, and for the batched cavities, the autotuner picks per n1
+    mesh runs tile option `option` with non-temporal LSRK register accesses
+    `stream_k` (1 / 0, -1 the default rule) -- the choices the autotuner makes in
+    the timed benchmark, given explicitly so that every counter pass profiles the
+    same kernel: under a counter pass the tuner's own timings are not
+    meaningful; with option -1, and for the batched cavities, the autotuner picks per n1
     segment (its timing launches of every option come first; the batch runs one
     persistent launch per segment and stage)."""
     import ctypes
@@ -194,6 +195,8 @@ def profile_run(nw, W, precision, steps, workload="c2", option=-1):
         s.uploadState(wl.states[0])
         if option < 0:
             s.tune()
+        else:
+            s.setSegmentOption(0, option, stream_k)
     else:
         s = nw.BatchSolver(precision=precision, use_graph=False)
         s.initialize(wl.meshes, wl.c0)
@@ -355,6 +358,8 @@ def main():
                     help="workload of --profile-run (single-mesh workloads)")
     ap.add_argument("--profile-option", type=int, default=-1,
                     help="tile option of --profile-run on a single mesh (-1: the autotuner's pick)")
+    ap.add_argument("--profile-stream", type=int, default=-1, choices=[-1, 0, 1],
+                    help="non-temporal k accesses of --profile-run with --profile-option (-1: default rule)")
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for rocprofv3 counter passes: the fused stage on the workload "
                          "plus a 256 MiB device copy as the HBM byte-count calibration")
@@ -369,7 +374,7 @@ def main():
 
     if args.profile_run:
         profile_run(nw, W, args.precision, steps=min(args.steps, 4), workload=args.profile_workload,
-                    option=args.profile_option)
+                    option=args.profile_option, stream_k=args.profile_stream)
         return
 
     dev = nw.device_info(0)
@@ -450,6 +455,7 @@ def main():
                    "precision": args.precision, "elements_per_workgroup": sinfo["epw"],
                    "tile_option": segs[0]["option"],
                    "volume_contractions": "matrix cores" if segs[0]["matrix_volume"] else "vector ALUs",
+                   "lsrk_register_nontemporal": segs[0]["stream_k"],
                    "device": dev["arch"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,

@@ -844,21 +844,43 @@ int Solver::tune() {
             const int st = launch_segment(sg, 1, cur, opt, 0, stream);
             if (st) return st;
         }
-        for (int rep = 0; rep < 2; ++rep)
-            for (int opt = 0; opt < kNumTileOptions; ++opt)
-                for (int str = 0; str < nstream; ++str) {
-                    if (fused_option_of(sg.n1, opt, prec) != opt) continue;
-                    NW_TRY_HIP(hipEventRecord(ev0, stream));
-                    for (int r = 0; r < 3; ++r) {
-                        const int st = launch_segment(sg, 1, cur, opt, str, stream);
-                        if (st) return st;
-                    }
-                    NW_TRY_HIP(hipEventRecord(ev1, stream));
-                    NW_TRY_HIP(hipEventSynchronize(ev1));
-                    float ms = 0.0f;
-                    NW_TRY_HIP(hipEventElapsedTime(&ms, ev0, ev1));
-                    best[opt][str] = std::min(best[opt][str], ms);
-                }
+        // candidates: distinct options x k policies
+        std::vector<std::pair<int, int>> cand;
+        for (int opt = 0; opt < kNumTileOptions; ++opt)
+            if (fused_option_of(sg.n1, opt, prec) == opt)
+                for (int str = 0; str < nstream; ++str) cand.push_back({opt, str});
+        auto timed = [&](int opt, int str, int n, float* ms) -> int {
+            NW_TRY_HIP(hipEventRecord(ev0, stream));
+            for (int r = 0; r < n; ++r) {
+                const int st = launch_segment(sg, 1, cur, opt, str, stream);
+                if (st) return st;
+            }
+            NW_TRY_HIP(hipEventRecord(ev1, stream));
+            NW_TRY_HIP(hipEventSynchronize(ev1));
+            NW_TRY_HIP(hipEventElapsedTime(ms, ev0, ev1));
+            return NW_OK;
+        };
+        // bring the clocks up first (after host-side set-up they ramp for tens
+        // of milliseconds, which would favour whichever candidate runs last):
+        // about 30 ms of launches
+        float one = 0.0f;
+        int st = timed(0, 0, 1, &one);
+        if (st) return st;
+        const int nsettle = std::max(2, std::min(400, static_cast<int>(30.0f / std::max(one, 1e-3f))));
+        if ((st = timed(0, 0, nsettle, &one))) return st;
+        one /= static_cast<float>(nsettle);
+        // three rounds, forward and backward in turn (a drift of the clocks
+        // then does not favour one end), best of the rounds per candidate;
+        // each measurement at least ~2 ms of launches (a small segment's
+        // launch takes tens of microseconds)
+        const int nlaunch = std::max(3, std::min(200, static_cast<int>(2.0f / std::max(one, 1e-3f))));
+        for (int rep = 0; rep < 3; ++rep)
+            for (size_t x = 0; x < cand.size(); ++x) {
+                const auto& c = cand[rep & 1 ? cand.size() - 1 - x : x];
+                float ms = 0.0f;
+                if ((st = timed(c.first, c.second, nlaunch, &ms))) return st;
+                best[c.first][c.second] = std::min(best[c.first][c.second], ms);
+            }
         sg.option = 0;
         sg.stream = 0;
         for (int opt = 0; opt < kNumTileOptions; ++opt)
