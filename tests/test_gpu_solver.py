@@ -210,6 +210,26 @@ def test_segment_choices_give_the_same_bits(name):
                 assert np.array_equal(got, ref), (prec, opt, sk)
 
 
+def test_tune_between_runs_keeps_the_state():
+    """tune() after graph-captured steps: its timing launches write only the
+    stage's output buffer and k (the state stays in the input buffer), and the
+    graphs are re-captured with the new choices -- the run continues with the
+    bits of an untuned run."""
+    m, o, c0 = _build("periodic_jitter_p7")
+    st0 = random_state(o, 4)
+    dt = o.auto_dt()
+    ref = run_product(m, st0, dt, 6, c0, "fp32", "fused", tile_option=0)
+    s = nw.DgSolver(precision="fp32", pipeline="fused", use_graph=True, tile_option=0)
+    s.initialize(m, c0)
+    s.uploadState(st0)
+    s.run(dt, 3)
+    s.tune()
+    s.run(dt, 3)
+    got = s.downloadState()
+    s.finalize()
+    assert np.array_equal(got, ref)
+
+
 def test_set_segment_option_rejects_bad_arguments():
     m, o, c0 = _build("media_pec_p5")
     s = nw.DgSolver(precision="fp32", pipeline="fused")
