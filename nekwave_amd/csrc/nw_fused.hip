@@ -38,23 +38,27 @@ template <typename R> __device__ __forceinline__ R gD(int n1, int idx);
 template <> __device__ __forceinline__ double gD<double>(int n1, int idx) { return c_gll.D[n1][idx]; }
 template <> __device__ __forceinline__ float gD<float>(int n1, int idx) { return c_Df[n1][idx]; }
 
-// Tangent vectors d x/dr, d x/ds, d x/dt of the trilinear map at (r, s, t).
+// Tangent vectors d x/dr, d x/ds, d x/dt of the trilinear map at (r, s, t), in
+// arithmetic T, from the 24 coefficients c[d * 8 + x] (ElemGeom order).
+template <typename T>
 struct Tangents {
-    double r[3], s[3], t[3];
+    T r[3], s[3], t[3];
 };
-__device__ __forceinline__ Tangents tangents(const ElemGeom* G, double r, double s, double t) {
-    Tangents T;
+template <typename T>
+__device__ __forceinline__ Tangents<T> tangents(const T* cf, T r, T s, T t) {
+    Tangents<T> G;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const double* c = G->c[d];
-        T.r[d] = c[1] + c[4] * s + c[5] * t + c[7] * (s * t);
-        T.s[d] = c[2] + c[4] * r + c[6] * t + c[7] * (r * t);
-        T.t[d] = c[3] + c[5] * r + c[6] * s + c[7] * (r * s);
+        const T* c = cf + 8 * d;
+        G.r[d] = c[1] + c[4] * s + c[5] * t + c[7] * (s * t);
+        G.s[d] = c[2] + c[4] * r + c[6] * t + c[7] * (r * t);
+        G.t[d] = c[3] + c[5] * r + c[6] * s + c[7] * (r * s);
     }
-    return T;
+    return G;
 }
 
-__device__ __forceinline__ void cross3(const double* a, const double* b, double* o) {
+template <typename T>
+__device__ __forceinline__ void cross3(const T* a, const T* b, T* o) {
     o[0] = a[1] * b[2] - a[2] * b[1];
     o[1] = a[2] * b[0] - a[0] * b[2];
     o[2] = a[0] * b[1] - a[1] * b[0];
@@ -240,22 +244,24 @@ __device__ __forceinline__ void norm_and_inverse(double n2, double& len, double&
     inv = len > 1e-14 ? 1.0 / len : 0.0;
 }
 
-// FP64 geometry at reference point (r, s, t) of a trilinear element: the
-// cofactors C = J J^-T (mesh.cpp:291-309 times J) and 1/J, rounded to R.  On an
-// affine element (c4..c7 exactly 0) every product with them is an exact zero,
-// so the result is the same bits at every node.
+// Geometry at reference point (r, s, t) of a trilinear element: the cofactors
+// C = J J^-T (mesh.cpp:291-309 times J) and 1/J, in the field precision R (FP64
+// fields: FP64 arithmetic; FP32 fields: FP32 arithmetic on the coefficients
+// rounded once, relative error ~1e-7, below the FP32 fields' own rounding).  On
+// an affine element (c4..c7 exactly 0) every product with them is an exact
+// zero, so the result is the same bits at every node.
 template <typename R>
-__device__ __forceinline__ void node_geometry(const ElemGeom* G, double r, double s, double t, R (&C)[10]) {
-    const Tangents T = tangents(G, r, s, t);
-    double Cr[3], Cs[3], Ct[3];
+__device__ __forceinline__ void node_geometry(const R* cf, R r, R s, R t, R (&C)[10]) {
+    const Tangents<R> T = tangents(cf, r, s, t);
+    R Cr[3], Cs[3], Ct[3];
     cross3(T.s, T.t, Cr);  // (rx, ry, rz) * J
     cross3(T.t, T.r, Cs);  // (sx, sy, sz) * J
     cross3(T.r, T.s, Ct);  // (tx, ty, tz) * J
-    const double det = T.r[0] * Cr[0] + T.r[1] * Cr[1] + T.r[2] * Cr[2];
-    C[0] = R(Cr[0]); C[1] = R(Cr[1]); C[2] = R(Cr[2]);
-    C[3] = R(Cs[0]); C[4] = R(Cs[1]); C[5] = R(Cs[2]);
-    C[6] = R(Ct[0]); C[7] = R(Ct[1]); C[8] = R(Ct[2]);
-    C[9] = recip(R(det));
+    const R det = T.r[0] * Cr[0] + T.r[1] * Cr[1] + T.r[2] * Cr[2];
+    C[0] = Cr[0]; C[1] = Cr[1]; C[2] = Cr[2];
+    C[3] = Cs[0]; C[4] = Cs[1]; C[5] = Cs[2];
+    C[6] = Ct[0]; C[7] = Ct[1]; C[8] = Ct[2];
+    C[9] = recip(det);
 }
 
 // Records of an affine segment: each element's geometry computed once (at the
@@ -266,11 +272,11 @@ __global__ void k_affine_records(double* rec, int nelt) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nelt) return;
     double* q = rec + size_t(e) * 26;
-    ElemGeom g;
+    R cf[24];
 #pragma unroll
-    for (int x = 0; x < 24; ++x) (&g.c[0][0])[x] = q[x];
+    for (int x = 0; x < 24; ++x) cf[x] = R(q[x]);
     R C[10];
-    node_geometry(&g, 0.0, 0.0, 0.0, C);
+    node_geometry(cf, R(0), R(0), R(0), C);
 #pragma unroll
     for (int x = 0; x < 10; ++x) q[x] = double(C[x]);
 }
@@ -315,6 +321,10 @@ void k_fused_stage(FusedArgs A) {
     __shared__ R sC[EPW * NF + SPARE][9];  // per face point: Nanson vector, minus-side trace (6)
     __shared__ R sD[N2];               // D, row-major: sD[i * N1 + l] = D_il
     __shared__ ElemRec sE[2][EPW];     // element records, double-buffered over groups
+    // FP32 fields: the records' 24 trilinear coefficients rounded to R once per
+    // group (the per-node geometry reads them); FP64 fields read sE itself
+    constexpr bool CFR = sizeof(R) == 4 && !AFF;
+    __shared__ R sEc[CFR ? 2 : 1][CFR ? EPW : 1][24];
     // the i = n1-1 plane of the last slot of the previous group (its pre-stage
     // fields), rows of 7 words so a wave's reads of consecutive rows do not conflict
     __shared__ R sP3[N2][7];
@@ -363,7 +373,7 @@ void k_fused_stage(FusedArgs A) {
     const unsigned kv = unsigned(384 * (tid >> 6) + (tid & 63)) * unsigned(sizeof(KR));  // lane's k offset in the group's chunks
 
     // loop-invariant: node coordinates, inverse mass weight
-    const double zr = c_gll.z[N1][i], zs = c_gll.z[N1][j], zt = c_gll.z[N1][k];
+    const R zr = R(c_gll.z[N1][i]), zs = R(c_gll.z[N1][j]), zt = R(c_gll.z[N1][k]);
     const R Winv = R(c_gll.iw[N1][i] * c_gll.iw[N1][j] * c_gll.iw[N1][k]);
     R wpq[FPT];  // w_p w_q of the thread's face point(s)
 #pragma unroll
@@ -448,6 +458,10 @@ void k_fused_stage(FusedArgs A) {
         for (int t = 0; t < GQ; ++t) {
             const int x = tid + t * NT;
             if (x < EPW * ERW) (&sE[buf][0].g.c[0][0])[x] = Q.w[t];
+            if constexpr (CFR) {
+                const int el = x / ERW, w = x - el * ERW;
+                if (x < EPW * ERW && w < 24) sEc[buf][el][w] = R(Q.w[t]);
+            }
         }
     };
     // gprev: the group this workgroup processes just before g (-1: none)
@@ -509,23 +523,25 @@ void k_fused_stage(FusedArgs A) {
     __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
     __syncthreads();
 
-    // FP64 geometry of the thread's node from an element record: cofactors
-    // C = J J^-T (mesh.cpp:291-309 times J) and 1/J, rounded to R.  A group's
-    // geometry is computed at the end of the previous iteration, where the
-    // waves otherwise wait for the prefetched fields to land.
-    // Affine segments (every element a parallelepiped) carry these ten values,
-    // precomputed once per element by k_affine_records with the same
+    // Geometry of the thread's node from an element record (node_geometry, in
+    // the field precision): cofactors C = J J^-T (mesh.cpp:291-309 times J) and
+    // 1/J.  A group's geometry is computed at the end of the previous
+    // iteration, where the waves otherwise wait for the prefetched fields to
+    // land.  Affine segments (every element a parallelepiped) carry these ten
+    // values, precomputed once per element by k_affine_records with the same
     // arithmetic, in the record's first ten words: read them instead.
-    auto geometry = [&](const ElemRec& ER, R(&C)[10]) {
+    auto geometry = [&](int buf, R(&C)[10]) {
         if constexpr (AFF) {
 #pragma unroll
-            for (int x = 0; x < 10; ++x) C[x] = R((&ER.g.c[0][0])[x]);
+            for (int x = 0; x < 10; ++x) C[x] = R((&sE[buf][es].g.c[0][0])[x]);
+        } else if constexpr (CFR) {
+            node_geometry(&sEc[buf][es][0], zr, zs, zt, C);
         } else {
-            node_geometry(&ER.g, zr, zs, zt, C);
+            node_geometry(&sE[buf][es].g.c[0][0], zr, zs, zt, C);
         }
     };
     R geo[10];
-    geometry(sE[0][es], geo);
+    geometry(0, geo);
 
     for (int t = 0; t <= tl; ++t) {
         const int g = gA;
@@ -857,7 +873,7 @@ void k_fused_stage(FusedArgs A) {
                 stk<0>(rK, kv, ks, kn);
             st6(rO, nd * UB, un);
         }
-        geometry(sE[(t + 1) & 1][es], geo);  // the next group's record, staged in this iteration
+        geometry((t + 1) & 1, geo);  // the next group's record, staged in this iteration
         cur = nxt;
         nb1 = nb2;
         rec1 = rec2;
