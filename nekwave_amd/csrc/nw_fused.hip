@@ -228,8 +228,7 @@ __device__ __forceinline__ void split4(const float* x, v4s& hi, v4s& lo) {
     lo = __builtin_bit_cast(v4s, (unsigned long long)l[0] | ((unsigned long long)l[1] << 32));
 }
 
-// 1/J of the FP64 Jacobian: in FP32 the hardware reciprocal (1 ulp) of the
-// rounded determinant; in FP64 the IEEE quotient.
+// 1/J: in FP32 the hardware reciprocal (1 ulp); in FP64 the IEEE quotient.
 __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double recip(double x) { return 1.0 / x; }
 // |N| and 1/|N| from |N|^2 with the reference's guard |N| > 1e-14
