@@ -769,11 +769,20 @@ void k_fused_stage(FusedArgs A) {
                             ls = NP;
                         }
                     }
-                    const bool lds = kind != kGather;
+                    // the plus trace as one block (kind is wave-uniform at n1 = 8:
+                    // one branch instead of one per component)
+                    R plus[6];
+                    if (kind != kGather) {
+#pragma unroll
+                        for (int c = 0; c < 6; ++c) plus[c] = lp[c * ls];
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < 6; ++c) plus[c] = cur.up[t][c];
+                    }
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
-                        jE[c] = (lds ? lp[c * ls] : cur.up[t][c]) - mine[c];
-                        jH[c] = (lds ? lp[(c + 3) * ls] : cur.up[t][c + 3]) - mine[c + 3];
+                        jE[c] = plus[c] - mine[c];
+                        jH[c] = plus[c + 3] - mine[c + 3];
                     }
                 }
                 // unit normal and area element (mesh.cpp:1032-1038)
