@@ -512,8 +512,10 @@ void k_fused_stage(FusedArgs A) {
         fetch_rec(gA, rec0);
         fetch_nbr(gB, nb1);
         fetch_rec(gB, rec1);
-        put_rec(rec0, 0);
+        // the fields first: staging the record waits for it, and the field
+        // reads then travel during that wait instead of after it
         fetch(gA, nb0, cur, -1);
+        put_rec(rec0, 0);
     }
     // Wait for the prologue's reads here: left in flight, the compiler's
     // memory-counter analysis merges them into the loop and then waits, at the
