@@ -52,10 +52,12 @@ static const double kRkB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.
 // of a box mesh).  Along a chain the kernel takes the face-3 traces of a group
 // from the plane its predecessor left in LDS instead of gathering them, so when
 // the mesh has chains every workgroup walks along them: equal chains are cut
-// into equal segments dealt round-robin in Morton (Z-order) order of their
-// first group's centroid; chains of different lengths (a batch: storage order)
-// are concatenated and cut into nseq pieces of nearly equal length, neighbouring
-// pieces on one XCD.  Without chains (single groups), the grid's iteration t
+// into equal segments (as many per chain as makes the busiest workgroup's share
+// smallest) dealt round-robin in Morton (Z-order) order of their first group's
+// centroid; chains of different lengths (a batch: storage order), and equal
+// chains whose segments cannot be dealt within 1.25x of the balanced share,
+// are concatenated and cut into nseq pieces of nearly equal length,
+// neighbouring pieces on one XCD.  Without chains (single groups), the grid's iteration t
 // covers positions t nseq .. t nseq + nseq - 1 of the Morton curve, a compact
 // block of elements whose face neighbours are read while still in L2.
 static bool uniform_chains(const std::vector<int>& cstart, int nchain) {
@@ -114,6 +116,28 @@ static void append_walk(const Mesh* m, const int4* fnb, int n1, int epw, int nel
             key[size_t(g)] = code;
         }
     }
+    // equal chains cut into d pieces each (d dividing the chain length): the
+    // busiest workgroup walks ceil(nchain d / nseq) pieces of lc / d groups;
+    // the d with the fewest groups on the busiest workgroup (the longest pieces
+    // among equals) -- a launch lasts as long as its busiest workgroup
+    int mseg = 0;
+    bool uniform = 2 * nchain <= ng && uniform_chains(cstart, nchain);
+    if (uniform) {
+        const int lc = cstart[1] - cstart[0];
+        long long best = -1;
+        for (int d = 1; d <= lc; ++d) {
+            if (lc % d) continue;
+            const long long load = ((long long)nchain * d + nseq - 1) / nseq * (lc / d);
+            if (best < 0 || load < best) {
+                best = load;
+                mseg = d;
+            }
+        }
+        // far from the balanced share (ceil(ng / nseq)): walk the chains as
+        // unequal ones, cut into nseq nearly equal pieces
+        const long long share = ((long long)ng + nseq - 1) / nseq;
+        if (4 * best > 5 * share) uniform = false;
+    }
     if (2 * nchain > ng) {
         // (almost) no chains: round-robin positions of the Morton curve (storage order for a batch)
         std::vector<int> curve(static_cast<size_t>(ng));
@@ -123,19 +147,12 @@ static void append_walk(const Mesh* m, const int4* fnb, int n1, int epw, int nel
             seq.push_back(static_cast<int>(order.size()) - obase);
             for (long long p = W; p < ng; p += nseq) order.push_back(curve[size_t(p)]);
         }
-    } else if (uniform_chains(cstart, nchain)) {
-        // equal chains (the x-lines of a box mesh): each cut into m equal segments,
-        // m the smallest divisor of the chain length giving every workgroup a
-        // segment; segments in Morton order dealt round-robin, so in every round
-        // the workgroups of one XCD walk neighbouring segments side by side and
-        // the y and z neighbours of a group are visited at the same time
+    } else if (uniform) {
+        // equal chains (the x-lines of a box mesh): each cut into mseg equal
+        // segments (chosen above); segments in Morton order dealt round-robin, so
+        // in every round the workgroups of one XCD walk neighbouring segments side
+        // by side and the y and z neighbours of a group are visited at the same time
         const int lc = cstart[1] - cstart[0];
-        int mseg = lc;
-        for (int d = 1; d <= lc; ++d)
-            if (lc % d == 0 && (long long)nchain * d >= nseq) {
-                mseg = d;
-                break;
-            }
         const int ls = lc / mseg;
         std::vector<int> sgs;  // first group of each segment
         for (int c = 0; c < nchain; ++c)
