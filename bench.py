@@ -176,6 +176,26 @@ def load_traffic(name):
     return None
 
 
+def profile_graph(nw, W, precision, steps, option, stream_k):
+    """The headline timing alone, for a kernel trace: the 16^3 workload with the
+    given tile option and k policy (no autotuner launches), a short clock-settling
+    run, 5 warm-up steps, then `steps` steps as captured graphs, timed with HIP
+    events like the benchmark; prints the event mean per launch for comparison
+    with the trace's mean kernel duration."""
+    wl = W.c2()
+    s = nw.DgSolver(precision=precision, pipeline="fused", use_graph=True, tile_option=option)
+    s.initialize(wl.meshes[0], wl.c0)
+    s.uploadState(wl.states[0])
+    s.setSegmentOption(0, option, stream_k)
+    settle(s, wl.dt, 0.1)
+    ms = s.timeRepeats(wl.dt, steps, 3, warmup=5)
+    info = s.info()
+    s.finalize()
+    per_launch = float(np.median(ms)) / (steps * info["kernels_per_step"])
+    print("profile graph: %d steps x 3, %.4f ms per step, event mean per launch %.2f us"
+          % (steps, float(np.median(ms)) / steps, 1e3 * per_launch), flush=True)
+
+
 def profile_run(nw, W, precision, steps, workload="c2", option=-1, stream_k=-1):
     """A few steps without graphs, so a kernel trace sees every launch.  A single
     mesh runs tile option `option` with non-temporal LSRK register accesses
@@ -360,6 +380,9 @@ def main():
                     help="tile option of --profile-run on a single mesh (-1: the autotuner's pick)")
     ap.add_argument("--profile-stream", type=int, default=-1, choices=[-1, 0, 1],
                     help="non-temporal k accesses of --profile-run with --profile-option (-1: default rule)")
+    ap.add_argument("--profile-graph", action="store_true",
+                    help="the headline timing alone (tile option --profile-option, k policy --profile-stream, "
+                         "no autotuner), for a kernel trace of its graph launches")
     ap.add_argument("--profile-run", action="store_true",
                     help="short run for rocprofv3 counter passes: the fused stage on the workload "
                          "plus a 256 MiB device copy as the HBM byte-count calibration")
@@ -372,6 +395,9 @@ def main():
     import nekwave_amd as nw
     from nekwave_amd import workloads as W
 
+    if args.profile_graph:
+        profile_graph(nw, W, args.precision, args.steps, max(args.profile_option, 0), args.profile_stream)
+        return
     if args.profile_run:
         profile_run(nw, W, args.precision, steps=min(args.steps, 4), workload=args.profile_workload,
                     option=args.profile_option, stream_k=args.profile_stream)
