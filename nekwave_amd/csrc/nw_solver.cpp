@@ -53,11 +53,10 @@ static const double kRkB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.
 // from the plane its predecessor left in LDS instead of gathering them, so when
 // the mesh has chains every workgroup walks along them: equal chains are cut
 // into equal segments (as many per chain as makes the busiest workgroup's share
-// smallest) dealt round-robin in Morton (Z-order) order of their first group's
-// centroid; chains of different lengths (a batch: storage order), and equal
-// chains whose segments cannot be dealt within 1.25x of the balanced share,
-// are concatenated and cut into nseq pieces of nearly equal length,
-// neighbouring pieces on one XCD.  Without chains (single groups), the grid's iteration t
+// smallest: the balanced share, in the longest segments that reach it) dealt
+// round-robin in Morton (Z-order) order of their first group's centroid;
+// chains of different lengths (a batch: storage order) are concatenated and cut
+// into nseq pieces of nearly equal length, neighbouring pieces on one XCD.  Without chains (single groups), the grid's iteration t
 // covers positions t nseq .. t nseq + nseq - 1 of the Morton curve, a compact
 // block of elements whose face neighbours are read while still in L2.
 static bool uniform_chains(const std::vector<int>& cstart, int nchain) {
@@ -119,9 +118,11 @@ static void append_walk(const Mesh* m, const int4* fnb, int n1, int epw, int nel
     // equal chains cut into d pieces each (d dividing the chain length): the
     // busiest workgroup walks ceil(nchain d / nseq) pieces of lc / d groups;
     // the d with the fewest groups on the busiest workgroup (the longest pieces
-    // among equals) -- a launch lasts as long as its busiest workgroup
+    // among equals) -- a launch lasts as long as its busiest workgroup.  d = lc
+    // always reaches the balanced share ceil(ng / nseq), so the choice never
+    // leaves a workgroup with more than that.
     int mseg = 0;
-    bool uniform = 2 * nchain <= ng && uniform_chains(cstart, nchain);
+    const bool uniform = 2 * nchain <= ng && uniform_chains(cstart, nchain);
     if (uniform) {
         const int lc = cstart[1] - cstart[0];
         long long best = -1;
@@ -133,10 +134,6 @@ static void append_walk(const Mesh* m, const int4* fnb, int n1, int epw, int nel
                 mseg = d;
             }
         }
-        // far from the balanced share (ceil(ng / nseq)): walk the chains as
-        // unequal ones, cut into nseq nearly equal pieces
-        const long long share = ((long long)ng + nseq - 1) / nseq;
-        if (4 * best > 5 * share) uniform = false;
     }
     if (2 * nchain > ng) {
         // (almost) no chains: round-robin positions of the Morton curve (storage order for a batch)
