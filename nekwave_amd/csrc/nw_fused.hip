@@ -1042,8 +1042,10 @@ int fused_option_of(int n1, int option, int prec) {
     return canonical_option(n1, clamp_option(option), prec == kFP64 ? 8 : 4);
 }
 
+bool fused_beyond_infinity_cache(long long ws_bytes) { return ws_bytes > (256ll << 20); }
+
 bool fused_stream_k(long long ws_bytes, int threads, int ksize) {
-    return ws_bytes > (256ll << 20) && ((threads % 64) * ksize) % 128 == 0;
+    return fused_beyond_infinity_cache(ws_bytes) && ((threads % 64) * ksize) % 128 == 0;
 }
 
 long long fused_k_chunks(int n1, int option, int prec, int nelt) {

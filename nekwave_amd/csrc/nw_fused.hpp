@@ -50,6 +50,8 @@ struct FusedArgs {
 // also covers whole cache lines of k (elements `ksize` bytes): a partly written
 // line streamed out is written again when its other part is.
 bool fused_stream_k(long long ws_bytes, int threads, int ksize);
+// Whether a stage's working set is larger than the 256 MiB Infinity Cache.
+bool fused_beyond_infinity_cache(long long ws_bytes);
 // k chunks (6 x 64 values each) of one group of `threads` threads, and of a
 // segment of `nelt` elements under a tile option: a step's stages all use one
 // option, so the k buffer holds, per segment, the most any option needs.

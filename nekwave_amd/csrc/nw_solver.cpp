@@ -839,7 +839,7 @@ int Solver::tune() {
     NW_TRY_HIP(hipSetDevice(device));
     // non-temporal k is a candidate only where the working set exceeds the
     // Infinity Cache (below it the hint only costs)
-    const int nstream = fused_stream_k(working_set_bytes(), 0, 1) ? 2 : 1;
+    const int nstream = fused_beyond_infinity_cache(working_set_bytes()) ? 2 : 1;
     for (Segment& sg : seg) {
         const TuneKey key{device, sg.n1, prec, inst.size() == 1 ? 1 : 0, sg.nelt};
         {
