@@ -84,6 +84,32 @@ def test_c2_fp32_vs_fp64_after_1000_steps(c2, precision, bound):
     assert err <= bound, err
 
 
+def test_short_walk_segments_match_oracle():
+    """A 16 x 16 x 12 jittered periodic mesh at n1 = 8, where the walk cuts the
+    x-chains into short segments (4 elements in FP64, 2 in FP32, so that no
+    workgroup walks more than the balanced share) and every segment's first
+    element takes its face-3 trace from another workgroup: FP64 matches the
+    oracle over 6 steps, FP32 stays within its bound of FP64 over 200."""
+    m = nw.Mesh(8)
+    m.createBoxMesh(16, 16, 12, 0, 1, 0, 1, 0, 1, True, True, True, jitter=0.1, seed=7)
+    o = po.Mesh.box(8, (16, 16, 12), (0, 1, 0, 1, 0, 1), (1, 1, 1), jitter=0.1, seed=7)
+    x, y, z = o.x, o.y, o.zc
+    r2 = (x - 0.5) ** 2 + (y - 0.5) ** 2 + (z - 0.5) ** 2
+    zero = np.zeros_like(x)
+    st = np.concatenate([zero, zero, np.exp(-r2 / 0.01), zero, zero, zero])
+    dt = o.auto_dt()
+    ref = oracle_run(o, st, dt, 6)
+    got = product_run(m, st, dt, 6, "fp64")
+    err = rel_l2(an.dV(o), got, ref)
+    print("16x16x12 fp64 6 steps relL2=%.3e" % err, flush=True)
+    assert err <= 1e-12, err
+    r64 = product_run(m, st, dt, 200, "fp64")
+    r32 = product_run(m, st, dt, 200, "fp32")
+    err = rel_l2(an.dV(o), r32, r64)
+    print("16x16x12 fp32 vs fp64 200 steps relL2=%.3e" % err, flush=True)
+    assert err <= 1e-5, err
+
+
 def test_c2_bf16x3_vs_fp64_after_1000_steps(c2):
     """The split-BF16 mode's stated bound: relative L2 <= 1e-4 against the FP64
     path after 1000 steps of the 16^3 workload."""
