@@ -211,7 +211,8 @@ int nw_mesh_locate_probes(const nw_mesh* m, int n, const double* xyz, int* elem,
  * non-temporal k accesses where the working set exceeds the 256 MiB Infinity
  * Cache -- and keep the fastest (runs by itself before the first step when
  * tile_option = -1).  Choices are kept per process, keyed by device, n1,
- * precision, element count and single mesh or batch. */
+ * precision, element count, single mesh or batch, and whether the working set
+ * exceeds the Infinity Cache. */
 int nw_dg_tune(nw_dg* s);
 /* n1, element count and chosen elements per workgroup of segment `seg`. */
 int nw_dg_segment_info(const nw_dg* s, int seg, int* n1, int* nelt, int* epw);

@@ -808,19 +808,22 @@ int Solver::build_graph(int parity) {
     return NW_OK;
 }
 
-// Tuned tile options kept for the life of the process, keyed by what decides
-// them: device, n1, precision, element count of the segment and whether it is a
-// single mesh (Morton walk) or a batch segment (storage order).  A later solver
-// of the same shape takes the choice without timing again.
+// Tuned tile options and k policies kept for the life of the process, keyed by
+// what decides them: device, n1, precision, element count of the segment,
+// whether it is a single mesh (Morton walk) or a batch segment (storage order),
+// and whether the working set exceeds the Infinity Cache (non-temporal k a
+// candidate).  A later solver of the same shape takes the choice without timing
+// again.
 namespace {
 struct TuneKey {
-    int device, n1, prec, single;
+    int device, n1, prec, single, streamable;
     long long nelt;
     bool operator<(const TuneKey& o) const {
         if (device != o.device) return device < o.device;
         if (n1 != o.n1) return n1 < o.n1;
         if (prec != o.prec) return prec < o.prec;
         if (single != o.single) return single < o.single;
+        if (streamable != o.streamable) return streamable < o.streamable;
         return nelt < o.nelt;
     }
 };
@@ -841,7 +844,7 @@ int Solver::tune() {
     // Infinity Cache (below it the hint only costs)
     const int nstream = fused_beyond_infinity_cache(working_set_bytes()) ? 2 : 1;
     for (Segment& sg : seg) {
-        const TuneKey key{device, sg.n1, prec, inst.size() == 1 ? 1 : 0, sg.nelt};
+        const TuneKey key{device, sg.n1, prec, inst.size() == 1 ? 1 : 0, nstream - 1, sg.nelt};
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
             auto it = g_tune.find(key);
