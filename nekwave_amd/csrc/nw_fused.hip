@@ -280,15 +280,15 @@ __global__ void k_affine_records(double* rec, int nelt) {
     for (int x = 0; x < 10; ++x) q[x] = double(C[x]);
 }
 
-// Persistent grid: in iteration t workgroup W takes the group at position
-// t * gstride + W of the group order, so one iteration of the grid covers a
-// contiguous window of the order (a compact block of elements when the order
-// follows a space-filling curve, keeping face neighbours' traces in L2) and
-// each XCD a contiguous part of that window.  HBM reads are issued one or two
+// Persistent grid: workgroup W walks its own sequence of element groups,
+// gorder[gseq[W] .. gseq[W+1]) (built on the host: along the x-chains of the
+// mesh, neighbouring sequences on one XCD, so face neighbours' traces are read
+// while still in that XCD's L2, and an x face whose neighbour is the previous
+// or next group in LDS needs no gather).  HBM reads are issued one or two
 // iterations before they are needed, so they travel while earlier groups
 // compute.  Per group:
-//   1. the node's face (Nanson) vectors from its FP64 geometry (computed
-//      during the previous iteration) and its field values go to LDS; barrier
+//   1. the node's face (Nanson) vectors from its geometry (computed during the
+//      previous iteration) and its field values go to LDS;           barrier
 //   2. the next group's reads are issued; volume: the three reference
 //      derivatives of each field from LDS folded straight into J * curl
 //      (matrix-core variant: derivatives to LDS, folded after the barrier);
