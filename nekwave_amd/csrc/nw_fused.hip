@@ -281,7 +281,7 @@ __global__ void k_affine_records(double* rec, int nelt) {
 }
 
 // Persistent grid: workgroup W walks its own sequence of element groups,
-// gorder[gseq[W] .. gseq[W+1]) (built on the host: along the x-chains of the
+// gorder[s0 .. s0 + tl] from its header (built on the host: along the x-chains of the
 // mesh, neighbouring sequences on one XCD, so face neighbours' traces are read
 // while still in that XCD's L2, and an x face whose neighbour is the previous
 // or next group in LDS needs no gather).  HBM reads are issued one or two
@@ -348,10 +348,13 @@ void k_fused_stage(FusedArgs A) {
     const int hb = blockIdx.x;
     const int W = (hb & 7) * A.wchunk + (hb >> 3);
     if (W >= A.nseq) return;  // uniform over the block, before any barrier
-    // this workgroup's groups: gorder[s0 .. s0 + tl], in that order
+    // this workgroup's groups: gorder[s0 .. s0 + tl], in that order; its header
+    // also holds the first three, so the first field reads wait on one scalar
+    // read instead of two in a row
     typedef const __attribute__((address_space(4))) int CInt;
-    const int s0 = ((CInt*)A.gseq)[W];
-    const int tl = ((CInt*)A.gseq)[W + 1] - s0 - 1;  // last iteration
+    CInt* const hd = (CInt*)A.ghead + kFusedHead * W;
+    const int s0 = hd[0];
+    const int tl = hd[1];  // last iteration
     if (tl < 0) return;
 
     const int tid = threadIdx.x;
@@ -504,7 +507,7 @@ void k_fused_stage(FusedArgs A) {
     Nbr nb1;
     Rec rec1;
     // groups of iterations t, t + 1, t + 2 (wave-uniform)
-    int gA = order_s(0), gB = order_s(1), gC = order_s(2);
+    int gA = hd[2], gB = hd[3], gC = hd[4];
     {
         Nbr nb0;
         Rec rec0;
@@ -960,7 +963,7 @@ hipError_t launch_one(const FusedArgs& a, hipStream_t s) {
     // the segment's k chunks (one per wave and group) lie inside the k buffer
     if (a.kbase < 0 || (a.kbase + (long long)b.ngroups * fused_k_chunks_per_group(NT)) * 64 > a.kstride)
         return hipErrorInvalidValue;
-    if (!a.gorder || !a.gseq || a.nseq <= 0 || a.nseq > b.ngroups) return hipErrorInvalidValue;
+    if (!a.gorder || !a.ghead || a.nseq <= 0 || a.nseq > b.ngroups) return hipErrorInvalidValue;
     b.wchunk = (a.nseq + 7) / 8;
     b.gstride = 8 * b.wchunk;  // workgroups past nseq return at once
     const dim3 grid(b.gstride), block(NT);

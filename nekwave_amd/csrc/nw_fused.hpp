@@ -12,6 +12,11 @@ enum Prec { kFP64 = 0, kFP32 = 1, kFP32K64 = 2, kBF16X3 = 3, kNumPrec = 4 };
 // Everything one launch of the fused stage kernel needs.  One launch covers a
 // contiguous range of elements that share n1 (a whole mesh, or one n1 segment
 // of a batched set of independent meshes).
+// Words of a workgroup's sequence header (FusedArgs::ghead): start s0 in the
+// group order, last iteration tl (-1: empty), the groups of iterations 0, 1, 2
+// (the last one repeated past tl), padding.
+constexpr int kFusedHead = 8;
+
 struct FusedArgs {
     const void* uin;          // node-major: the 6 values (R) of each node consecutive; ghosts after npts
     void* uout;               // same layout
@@ -27,7 +32,8 @@ struct FusedArgs {
     int gstride;              // workgroups of the persistent grid; set by the launcher
     int wchunk;               // workgroups per XCD (XCD-aware remap); set by the launcher
     const int* gorder;        // ngroups entries: the groups in visiting order (a permutation)
-    const int* gseq;          // nseq + 1 offsets into gorder: workgroup W visits gorder[gseq[W] .. gseq[W+1])
+    const int* ghead;         // kFusedHead words per workgroup W: s0, tl, the first three groups;
+                              // W visits gorder[s0 .. s0 + tl]
     int nseq;                 // workgroup sequences (<= the resident workgroups, fused_resident)
     const ElemGeom* geom;     // per element, segment-local
     const double* rec;        // per element, segment-local: 24 trilinear coefficients, 1/eps, 1/mu

@@ -407,7 +407,7 @@ int Solver::init(const Mesh* const* meshes, int count, double c0_, const nw_opts
             }
         }
         if ((st = upload_vec(&d_fnb, fnb))) return st;
-        std::vector<int> gorder, gseq;
+        std::vector<int> gorder, ghead;
         for (int opt = 0; opt < kNumTileOptions; ++opt)
             for (Segment& sg : seg) {
                 const int epw = fused_epw(sg.n1, opt, prec);
@@ -417,14 +417,23 @@ int Solver::init(const Mesh* const* meshes, int count, double c0_, const nw_opts
                     return NW_ERR_HIP;
                 }
                 sg.goff[opt] = static_cast<long long>(gorder.size());
-                sg.soff[opt] = static_cast<long long>(gseq.size());
-                const size_t before = gseq.size();
+                sg.soff[opt] = static_cast<long long>(ghead.size());
+                std::vector<int> seq;
                 append_walk(count == 1 ? meshes[0] : nullptr, fnb.data() + size_t(sg.elem_base) * 6, sg.n1, epw,
-                            sg.nelt, sg.node_base, resident, gorder, gseq);
-                sg.nseq[opt] = static_cast<int>(gseq.size() - before) - 1;
+                            sg.nelt, sg.node_base, resident, gorder, seq);
+                sg.nseq[opt] = static_cast<int>(seq.size()) - 1;
+                // per workgroup sequence: its start, last iteration and first
+                // three groups (kFusedHead words), one scalar read for the kernel
+                const size_t go = static_cast<size_t>(sg.goff[opt]);
+                for (int W = 0; W < sg.nseq[opt]; ++W) {
+                    const int s0 = seq[size_t(W)], tl = seq[size_t(W) + 1] - s0 - 1;
+                    int head[kFusedHead] = {s0, tl, 0, 0, 0, 0, 0, 0};
+                    for (int t = 0; t < 3 && tl >= 0; ++t) head[2 + t] = gorder[go + size_t(s0 + std::min(t, tl))];
+                    ghead.insert(ghead.end(), head, head + kFusedHead);
+                }
             }
         if ((st = upload_vec(&d_gorder, gorder))) return st;
-        if ((st = upload_vec(&d_gseq, gseq))) return st;
+        if ((st = upload_vec(&d_ghead, ghead))) return st;
         // flux weights per element face from the two media (the plus side of a
         // mirror face is the element itself; of a cut face, its ghost medium)
         std::vector<double> w64(size_t(nelt) * 24);
@@ -586,7 +595,7 @@ int Solver::launch_segment(const Segment& sg, int s, int parity, int option, int
     a.gmat = d_gmat;
     a.fnb = d_fnb + size_t(sg.elem_base) * 6;
     a.gorder = d_gorder + sg.goff[option];
-    a.gseq = d_gseq + sg.soff[option];
+    a.ghead = d_ghead + sg.soff[option];
     a.nseq = sg.nseq[option];
     a.dt = d_dt;
     a.a = kRkA[s];

@@ -67,7 +67,7 @@ struct Solver {
     void* d_fw = nullptr;  // flux weights per element face, 4 values of the field type
     double* d_rec = nullptr;  // per element: 24 trilinear coefficients, 1/eps, 1/mu
     int* d_gorder = nullptr;  // per segment and tile option: the order the fused kernel visits groups in
-    int* d_gseq = nullptr;    // per segment and tile option: workgroup sequence offsets into it
+    int* d_ghead = nullptr;   // per segment and tile option: kFusedHead words per workgroup sequence
     // plain pipeline (reference residency, single mesh)
     double *d_state = nullptr, *d_kk = nullptr, *d_rhs = nullptr, *d_D = nullptr, *d_w3 = nullptr;
     double* d_met[13] = {};  // x y z jac rx sx tx ry sy ty rz sz tz
