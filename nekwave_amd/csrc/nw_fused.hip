@@ -342,17 +342,15 @@ void k_fused_stage(FusedArgs A) {
     __shared__ R sA[MX ? 2 : 1][MX ? 6 : 1][MX ? NP : 1];
     __shared__ R sAt[MX ? 6 : 1][MX ? 9 * N2 : 1];
 
-    // XCD-aware remap: the hardware deals workgroups round-robin over the 8
-    // XCDs; logical workgroups are contiguous per XCD so each XCD's share of a
-    // window, and with it most face neighbours, stays in that XCD's L2.
-    const int hb = blockIdx.x;
-    const int W = (hb & 7) * A.wchunk + (hb >> 3);
-    if (W >= A.nseq) return;  // uniform over the block, before any barrier
-    // this workgroup's groups: gorder[s0 .. s0 + tl], in that order; its header
-    // also holds the first three, so the first field reads wait on one scalar
-    // read instead of two in a row
+    // This workgroup's groups: gorder[s0 .. s0 + tl], in that order.  The
+    // headers are stored by hardware workgroup id, already remapped XCD-aware
+    // (fused_sequence_of_block: the hardware deals workgroups round-robin over
+    // the 8 XCDs, and neighbouring sequences, with most face neighbours, go to
+    // one XCD's L2); a header also holds the first three groups, so the first
+    // field reads wait on one scalar read.  Workgroups past the sequences have
+    // tl = -1 and return (uniform over the block, before any barrier).
     typedef const __attribute__((address_space(4))) int CInt;
-    CInt* const hd = (CInt*)A.ghead + kFusedHead * W;
+    CInt* const hd = (CInt*)A.ghead + kFusedHead * blockIdx.x;
     const int s0 = hd[0];
     const int tl = hd[1];  // last iteration
     if (tl < 0) return;
@@ -964,8 +962,7 @@ hipError_t launch_one(const FusedArgs& a, hipStream_t s) {
     if (a.kbase < 0 || (a.kbase + (long long)b.ngroups * fused_k_chunks_per_group(NT)) * 64 > a.kstride)
         return hipErrorInvalidValue;
     if (!a.gorder || !a.ghead || a.nseq <= 0 || a.nseq > b.ngroups) return hipErrorInvalidValue;
-    b.wchunk = (a.nseq + 7) / 8;
-    b.gstride = 8 * b.wchunk;  // workgroups past nseq return at once
+    b.gstride = fused_grid(a.nseq);  // headers past the sequences return at once
     const dim3 grid(b.gstride), block(NT);
     if (a.affine) {
         if (a.first_stage)

@@ -16,6 +16,12 @@ enum Prec { kFP64 = 0, kFP32 = 1, kFP32K64 = 2, kBF16X3 = 3, kNumPrec = 4 };
 // group order, last iteration tl (-1: empty), the groups of iterations 0, 1, 2
 // (the last one repeated past tl), padding.
 constexpr int kFusedHead = 8;
+// The persistent grid of nseq sequences (a multiple of the 8 XCDs), and the
+// sequence hardware workgroup hb runs: the hardware deals workgroups
+// round-robin over the XCDs, so sequences W .. W + nseq/8 go to one XCD
+// (>= nseq: none).  The headers are stored in hardware workgroup order.
+constexpr int fused_grid(int nseq) { return 8 * ((nseq + 7) / 8); }
+constexpr int fused_sequence_of_block(int hb, int nseq) { return (hb & 7) * ((nseq + 7) / 8) + (hb >> 3); }
 
 struct FusedArgs {
     const void* uin;          // node-major: the 6 values (R) of each node consecutive; ghosts after npts
@@ -30,10 +36,9 @@ struct FusedArgs {
     int elem_base;            // global index of the segment's first element (into mat)
     int ngroups;              // element groups (EPW elements each); set by the launcher
     int gstride;              // workgroups of the persistent grid; set by the launcher
-    int wchunk;               // workgroups per XCD (XCD-aware remap); set by the launcher
     const int* gorder;        // ngroups entries: the groups in visiting order (a permutation)
-    const int* ghead;         // kFusedHead words per workgroup W: s0, tl, the first three groups;
-                              // W visits gorder[s0 .. s0 + tl]
+    const int* ghead;         // kFusedHead words per hardware workgroup (fused_grid(nseq) of them):
+                              // s0, tl, the first three groups; it visits gorder[s0 .. s0 + tl]
     int nseq;                 // workgroup sequences (<= the resident workgroups, fused_resident)
     const ElemGeom* geom;     // per element, segment-local
     const double* rec;        // per element, segment-local: 24 trilinear coefficients, 1/eps, 1/mu

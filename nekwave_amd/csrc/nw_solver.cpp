@@ -422,13 +422,20 @@ int Solver::init(const Mesh* const* meshes, int count, double c0_, const nw_opts
                 append_walk(count == 1 ? meshes[0] : nullptr, fnb.data() + size_t(sg.elem_base) * 6, sg.n1, epw,
                             sg.nelt, sg.node_base, resident, gorder, seq);
                 sg.nseq[opt] = static_cast<int>(seq.size()) - 1;
-                // per workgroup sequence: its start, last iteration and first
-                // three groups (kFusedHead words), one scalar read for the kernel
+                // per hardware workgroup of the grid, the header of the sequence
+                // it runs: start, last iteration and first three groups
+                // (kFusedHead words; tl = -1: none), one scalar read for the kernel
                 const size_t go = static_cast<size_t>(sg.goff[opt]);
-                for (int W = 0; W < sg.nseq[opt]; ++W) {
-                    const int s0 = seq[size_t(W)], tl = seq[size_t(W) + 1] - s0 - 1;
-                    int head[kFusedHead] = {s0, tl, 0, 0, 0, 0, 0, 0};
-                    for (int t = 0; t < 3 && tl >= 0; ++t) head[2 + t] = gorder[go + size_t(s0 + std::min(t, tl))];
+                const int ns = sg.nseq[opt];
+                for (int hb = 0; hb < fused_grid(ns); ++hb) {
+                    const int W = fused_sequence_of_block(hb, ns);
+                    int head[kFusedHead] = {0, -1, 0, 0, 0, 0, 0, 0};
+                    if (W < ns) {
+                        const int s0 = seq[size_t(W)], tl = seq[size_t(W) + 1] - s0 - 1;
+                        head[0] = s0;
+                        head[1] = tl;
+                        for (int t = 0; t < 3 && tl >= 0; ++t) head[2 + t] = gorder[go + size_t(s0 + std::min(t, tl))];
+                    }
                     ghead.insert(ghead.end(), head, head + kFusedHead);
                 }
             }
