@@ -132,8 +132,10 @@ typedef __attribute__((ext_vector_type(2))) double f64x2;
 // 32-bit lanes of the loaded vector before the reinterpretation let the
 // compiler narrow the access to its first lane.)
 // CP: the access's cache-policy bits (kNT: non-temporal -- no reuse expected,
-// so the line is a first candidate for eviction from L2).
+// so the line is a first candidate for eviction from L2; kSC1: system scope --
+// a store is written through L2 instead of left dirty in it).
 constexpr int kNT = 2;
+constexpr int kSC1 = 16;
 template <int CP = 0>
 __device__ __forceinline__ void ld6(__amdgpu_buffer_rsrc_t r, unsigned vo, float (&o)[6]) {
     const f32x4 a = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, vo, 0, CP));
@@ -878,10 +880,12 @@ void k_fused_stage(FusedArgs A) {
                 kn[c] = FIRST ? KR(dt) * KR(rhs) : KR(A.a) * kold[c] + KR(dt) * KR(rhs);
                 un[c] = R(KR(sU[es][c][m]) + KR(A.b) * kn[c]);
             }
+            // written through L2 (whole lines): the kernel boundary, which writes
+            // every dirty L2 line back for the other XCDs, has k's lines off its list
             if (A.stream_k)
                 stk<kNT>(rK, kv, ks, kn);
             else
-                stk<0>(rK, kv, ks, kn);
+                stk<kSC1>(rK, kv, ks, kn);
             st6(rO, nd * UB, un);
         }
         geometry((t + 1) & 1, geo);  // the next group's record, staged in this iteration
