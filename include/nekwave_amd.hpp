@@ -185,6 +185,19 @@ class DgSolver {
         return out;
     }
     nw_opts& options() { return opts_; }
+    // Extension: launch-shape autotuning (nw_dg_tune, nw_dg_segment_option,
+    // nw_dg_set_segment_option).  Results never depend on the choices.
+    void tune() { check(nw_dg_tune(s_)); }
+    void segmentOption(int seg, int* option, bool* matrixVolume = nullptr, bool* streamK = nullptr) const {
+        int o = 0, mv = 0, sk = 0;
+        check(nw_dg_segment_option(s_, seg, &o, &mv, &sk));
+        if (option) *option = o;
+        if (matrixVolume) *matrixVolume = mv != 0;
+        if (streamK) *streamK = sk != 0;
+    }
+    void setSegmentOption(int seg, int option, int streamK = -1) {
+        check(nw_dg_set_segment_option(s_, seg, option, streamK));
+    }
 
    private:
     nw_opts opts_;
