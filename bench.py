@@ -9,15 +9,23 @@ nodes/element, 2,097,152 nodes x 6 fields), Gaussian pulse, upwind flux,
 LSRK45 at the reference's automatic dt.  A step is one LSRK45 step (5 fused
 stage launches) over the whole mesh with the state resident in HBM.
 
-Prints one JSON line: DOF-updates/s of the FP32 fused path (value), the
-roofline of the fused stage kernel (algorithmic bytes per launch / mean launch
-time from HIP events on the solver's stream), the FP64 fused and FP64
-reference-decomposed ("plain") rates, the FP32 relative L2 error against the
-FP64 path after 1000 steps with the energy drift of both, five repeats of the
-timed loop, the PEC-cube TM110 case (relative L2 to the analytic mode, energy
-drift), the other workloads (c3 PEC 32^3 with media, c4 256 batched cavities,
-c5 64^3 share with the halo stand-in) and the CPU baseline (the oracle's FP64
-step on the host cores, bounded sample, and one step on one thread).
+Prints one JSON line: DOF-updates/s of the FP32 fused path (value: the median
+of five back-to-back timings of exactly K steps, after an untimed clock-settling
+run and W warm-up steps), the roofline of the fused stage kernel (algorithmic
+bytes per launch / mean launch time from HIP events on the solver's stream), the
+FP64 fused and FP64 reference-decomposed ("plain") rates, FP32 fields with FP64
+LSRK registers, the FP32 relative L2 error against the FP64 path after 1000 steps
+with the energy drift of both, the PEC-cube TM110 case (relative L2 to the
+analytic mode, energy drift), the Case loop with on-device probes against the
+download-every-step pattern, the other workloads with their stated errors (c3
+PEC 32^3 with media, FP32 and split BF16; c4 256 batched cavities; c5 64^3 share
+with the halo stand-in, FP32 and FP64) and the CPU baseline (the oracle's FP64
+step on the host's threads, a bounded sample, one thread, and the full TM110
+run).
+
+Profiling modes (short runs for rocprofv3): --profile-run (a few steps without
+graphs, then a 256 MiB copy as the byte-count calibration) and --profile-graph
+(the headline timing alone, with the tile option and k policy given).
 """
 import argparse
 import json
