@@ -25,7 +25,8 @@ run).
 
 Profiling modes (short runs for rocprofv3): --profile-run (a few steps without
 graphs, then a 256 MiB copy as the byte-count calibration) and --profile-graph
-(the headline timing alone, with the tile option and k policy given).
+(one single-mesh workload's graph timing alone, with the tile option and k policy
+given; --profile-workload picks the workload of either).
 """
 import argparse
 import json
@@ -184,13 +185,16 @@ def load_traffic(name):
     return None
 
 
-def profile_graph(nw, W, precision, steps, option, stream_k):
-    """The headline timing alone, for a kernel trace: the 16^3 workload with the
-    given tile option and k policy (no autotuner launches), a short clock-settling
-    run, 5 warm-up steps, then `steps` steps as captured graphs, timed with HIP
-    events like the benchmark; prints the event mean per launch for comparison
-    with the trace's mean kernel duration."""
-    wl = W.c2()
+def profile_graph(nw, W, precision, steps, option, stream_k, workload="c2"):
+    """The benchmark's timing of one single-mesh workload alone, for a kernel
+    trace: the given tile option and k policy (no autotuner launches), a short
+    clock-settling run, 5 warm-up steps, then `steps` steps as captured graphs,
+    timed with HIP events like the benchmark; prints the event mean per launch
+    (every kernel of a step: with the halo stand-in, pack and unpack launches
+    too) for comparison with the trace's mean kernel durations."""
+    wl = getattr(W, workload)()
+    if len(wl.meshes) != 1:
+        raise SystemExit("--profile-graph takes a single-mesh workload (c2, c3, c5)")
     s = nw.DgSolver(precision=precision, pipeline="fused", use_graph=True, tile_option=option)
     s.initialize(wl.meshes[0], wl.c0)
     s.uploadState(wl.states[0])
@@ -200,8 +204,8 @@ def profile_graph(nw, W, precision, steps, option, stream_k):
     info = s.info()
     s.finalize()
     per_launch = float(np.median(ms)) / (steps * info["kernels_per_step"])
-    print("profile graph: %d steps x 3, %.4f ms per step, event mean per launch %.2f us"
-          % (steps, float(np.median(ms)) / steps, 1e3 * per_launch), flush=True)
+    print("profile graph %s: %d steps x 3, %.4f ms per step, %d launches per step, event mean per launch %.2f us"
+          % (workload, steps, float(np.median(ms)) / steps, info["kernels_per_step"], 1e3 * per_launch), flush=True)
 
 
 def profile_run(nw, W, precision, steps, workload="c2", option=-1, stream_k=-1):
@@ -383,7 +387,7 @@ def main():
     ap.add_argument("--accuracy-steps", type=int, default=1000)
     ap.add_argument("--configs", default="c3,c4,c5", help="other workloads to time (comma list, or 'none')")
     ap.add_argument("--profile-workload", default="c2", choices=["c2", "c3", "c4", "c5"],
-                    help="workload of --profile-run (single-mesh workloads)")
+                    help="workload of --profile-run and --profile-graph (the latter: single-mesh workloads)")
     ap.add_argument("--profile-option", type=int, default=-1,
                     help="tile option of --profile-run on a single mesh (-1: the autotuner's pick)")
     ap.add_argument("--profile-stream", type=int, default=-1, choices=[-1, 0, 1],
@@ -404,7 +408,8 @@ def main():
     from nekwave_amd import workloads as W
 
     if args.profile_graph:
-        profile_graph(nw, W, args.precision, args.steps, max(args.profile_option, 0), args.profile_stream)
+        profile_graph(nw, W, args.precision, args.steps, max(args.profile_option, 0), args.profile_stream,
+                      workload=args.profile_workload)
         return
     if args.profile_run:
         profile_run(nw, W, args.precision, steps=min(args.steps, 4), workload=args.profile_workload,
