@@ -259,6 +259,54 @@ static void StepLimit() {
     std::printf("  PASSED\n");
 }
 
+// Extension API of the C++ header: every launch shape (tile option, LSRK
+// register policy) and the autotuner's choice give the bits of tile option 0.
+static void LaunchShapesGiveTheSameBits() {
+    std::printf("[RUN] LaunchShapesGiveTheSameBits\n");
+    const int N = 6;
+    Mesh mesh(N, 1);
+    mesh.createBoxMesh(4, 3, 2, 0.0, 1.0, 0.0, 0.8, 0.0, 0.5);
+    const int npts = mesh.getTotalPoints();
+    const auto x = mesh.getCoordX(), y = mesh.getCoordY(), z = mesh.getCoordZ();
+    StateVector init(6 * size_t(npts));
+    for (int c = 0; c < 6; ++c)
+        for (int i = 0; i < npts; ++i) init[size_t(c) * npts + i] = std::sin(3.0 * x[i] + (c + 1) * y[i]) * std::cos(2.0 * z[i] - c);
+    const double dt = 0.002;
+    StateVector ref;
+    for (int run = 0; run < 7; ++run) {  // options 0..2 x k policy 0/1, then the tuner's pick
+        nw_opts o;
+        nw_opts_default(&o);
+        o.precision = g_prec;
+        DgSolver solver(o);
+        EXPECT_TRUE(solver.initialize(mesh, 1.0));
+        solver.uploadState(init.data(), init.size());
+        const int opt = run / 2, sk = run % 2;
+        if (run < 6) {
+            solver.setSegmentOption(0, opt, sk);
+            int got = -1;
+            bool gotk = !sk;
+            solver.segmentOption(0, &got, nullptr, &gotk);
+            EXPECT_TRUE(got == opt && gotk == (sk != 0));
+        } else {
+            solver.tune();
+            int got = -1;
+            solver.segmentOption(0, &got);
+            EXPECT_TRUE(got >= 0 && got < 3);
+        }
+        solver.run(dt, 5);
+        StateVector out(6 * size_t(npts));
+        solver.downloadState(out.data(), out.size());
+        solver.finalize();
+        if (run == 0) {
+            ref = out;
+            EXPECT_TRUE(out != init);
+        } else {
+            EXPECT_TRUE(out == ref);
+        }
+    }
+    std::printf("  PASSED\n");
+}
+
 int main(int argc, char** argv) {
     if (argc > 1 && std::strcmp(argv[1], "host") == 0) {  // host-side checks only (no device)
         try {
@@ -278,6 +326,7 @@ int main(int argc, char** argv) {
         MaxwellStabilityTest();
         BuiltinBlochIc();
         BoxPecCase();
+        LaunchShapesGiveTheSameBits();
     } catch (const Error& e) {
         std::fprintf(stderr, "error %d: %s\n", e.status(), e.what());
         return 2;
