@@ -25,8 +25,9 @@ run).
 
 Profiling modes (short runs for rocprofv3): --profile-run (a few steps without
 graphs, then a 256 MiB copy as the byte-count calibration) and --profile-graph
-(one single-mesh workload's graph timing alone, with the tile option and k policy
-given; --profile-workload picks the workload of either).
+(one workload's graph timing alone, with the tile option and k policy given for a
+single mesh, tuned for the batch; --profile-workload picks the workload of either,
+--profile-groups-per-cu caps the fused launches' workgroups per CU).
 """
 import argparse
 import json
@@ -185,20 +186,29 @@ def load_traffic(name):
     return None
 
 
-def profile_graph(nw, W, precision, steps, option, stream_k, workload="c2"):
-    """The benchmark's timing of one single-mesh workload alone, for a kernel
-    trace: the given tile option and k policy (no autotuner launches), a short
-    clock-settling run, 5 warm-up steps, then `steps` steps as captured graphs,
-    timed with HIP events like the benchmark; prints the event mean per launch
-    (every kernel of a step: with the halo stand-in, pack and unpack launches
-    too) for comparison with the trace's mean kernel durations."""
+def profile_graph(nw, W, precision, steps, option, stream_k, workload="c2", groups_per_cu=0):
+    """The benchmark's timing of one workload alone, for a kernel trace: a single
+    mesh with the given tile option and k policy (no autotuner launches), the
+    batched cavities with the autotuner's per-segment picks (its timing launches
+    come first); a short clock-settling run, 5 warm-up steps, then `steps` steps
+    as captured graphs, timed with HIP events like the benchmark; prints the
+    event mean per launch (every kernel of a step: with the halo stand-in, pack
+    and unpack launches too) for comparison with the trace's mean kernel
+    durations.  groups_per_cu caps the fused launches' workgroups per CU."""
     wl = getattr(W, workload)()
-    if len(wl.meshes) != 1:
-        raise SystemExit("--profile-graph takes a single-mesh workload (c2, c3, c5)")
-    s = nw.DgSolver(precision=precision, pipeline="fused", use_graph=True, tile_option=option)
-    s.initialize(wl.meshes[0], wl.c0)
-    s.uploadState(wl.states[0])
-    s.setSegmentOption(0, option, stream_k)
+    if len(wl.meshes) == 1:
+        s = nw.DgSolver(precision=precision, pipeline="fused", use_graph=True, tile_option=option,
+                        groups_per_cu=groups_per_cu)
+        s.initialize(wl.meshes[0], wl.c0)
+        s.uploadState(wl.states[0])
+        s.setSegmentOption(0, option, stream_k)
+    else:
+        s = nw.BatchSolver(precision=precision, use_graph=True, groups_per_cu=groups_per_cu)
+        s.initialize(wl.meshes, wl.c0)
+        for x, st in enumerate(wl.states):
+            s.uploadState(x, st)
+        s.tune()
+        print("segments:", s.segments(), flush=True)
     settle(s, wl.dt, 0.1)
     ms = s.timeRepeats(wl.dt, steps, 3, warmup=5)
     info = s.info()
@@ -387,7 +397,9 @@ def main():
     ap.add_argument("--accuracy-steps", type=int, default=1000)
     ap.add_argument("--configs", default="c3,c4,c5", help="other workloads to time (comma list, or 'none')")
     ap.add_argument("--profile-workload", default="c2", choices=["c2", "c3", "c4", "c5"],
-                    help="workload of --profile-run and --profile-graph (the latter: single-mesh workloads)")
+                    help="workload of --profile-run and --profile-graph")
+    ap.add_argument("--profile-groups-per-cu", type=int, default=0,
+                    help="cap on the fused launches' workgroups per CU in --profile-graph (0: as many as fit)")
     ap.add_argument("--profile-option", type=int, default=-1,
                     help="tile option of --profile-run on a single mesh (-1: the autotuner's pick)")
     ap.add_argument("--profile-stream", type=int, default=-1, choices=[-1, 0, 1],
@@ -409,7 +421,7 @@ def main():
 
     if args.profile_graph:
         profile_graph(nw, W, args.precision, args.steps, max(args.profile_option, 0), args.profile_stream,
-                      workload=args.profile_workload)
+                      workload=args.profile_workload, groups_per_cu=args.profile_groups_per_cu)
         return
     if args.profile_run:
         profile_run(nw, W, args.precision, steps=min(args.steps, 4), workload=args.profile_workload,

@@ -138,6 +138,10 @@ typedef struct nw_opts {
     int device;       /* HIP device ordinal (default 0) */
     int tile_option;  /* fused tile option 0/1/2 (elements per workgroup; at n1 = 8 in FP32,
                        * 1 = matrix-core volume), -1 = tuned default */
+    int groups_per_cu; /* extension, no counterpart in the reference: at most this many
+                        * workgroups of a fused launch per CU (0 = as many as fit, the
+                        * default), leaving room to other work on the device.  Results do
+                        * not depend on it. */
 } nw_opts;
 void nw_opts_default(nw_opts* o);
 
@@ -211,8 +215,8 @@ int nw_mesh_locate_probes(const nw_mesh* m, int n, const double* xyz, int* elem,
  * non-temporal k accesses where the working set exceeds the 256 MiB Infinity
  * Cache -- and keep the fastest (runs by itself before the first step when
  * tile_option = -1).  Choices are kept per process, keyed by device, n1,
- * precision, element count, single mesh or batch, and whether the working set
- * exceeds the Infinity Cache. */
+ * precision, element count, single mesh or batch, whether the working set
+ * exceeds the Infinity Cache, and groups_per_cu. */
 int nw_dg_tune(nw_dg* s);
 /* n1, element count and chosen elements per workgroup of segment `seg`. */
 int nw_dg_segment_info(const nw_dg* s, int seg, int* n1, int* nelt, int* epw);

@@ -241,9 +241,12 @@ class Mesh:
 
 class DgSolver:
     """Reference DgSolver.  Extensions: precision ('fp64', 'fp32', 'fp32_k64'),
-    pipeline ('fused' or 'plain' = the reference's 7-kernel stage), graphs."""
+    pipeline ('fused' or 'plain' = the reference's 7-kernel stage), graphs, the
+    fused tile option (-1: tuned) and groups_per_cu, a cap on the fused launch's
+    workgroups per CU (0: as many as fit)."""
 
-    def __init__(self, precision="fp64", pipeline="fused", use_graph=True, device=0, tile_option=-1):
+    def __init__(self, precision="fp64", pipeline="fused", use_graph=True, device=0, tile_option=-1,
+                 groups_per_cu=0):
         self.opts = _lib.Opts()
         lib().nw_opts_default(C.byref(self.opts))
         self.opts.precision = _lib.PREC[precision]
@@ -251,6 +254,7 @@ class DgSolver:
         self.opts.use_graph = int(bool(use_graph))
         self.opts.device = int(device)
         self.opts.tile_option = int(tile_option)
+        self.opts.groups_per_cu = int(groups_per_cu)
         self._h = None
         self.npts = 0
 
@@ -469,9 +473,9 @@ class BatchSolver(DgSolver):
     another.  Instances are addressed by their index in the list given to
     initialize()."""
 
-    def __init__(self, precision="fp32", use_graph=True, device=0, tile_option=-1):
+    def __init__(self, precision="fp32", use_graph=True, device=0, tile_option=-1, groups_per_cu=0):
         super().__init__(precision=precision, pipeline="fused", use_graph=use_graph, device=device,
-                         tile_option=tile_option)
+                         tile_option=tile_option, groups_per_cu=groups_per_cu)
         self.sizes = []
 
     def initialize(self, meshes, c0):

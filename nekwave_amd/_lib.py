@@ -31,7 +31,7 @@ class MeshInfo(C.Structure):
 
 class Opts(C.Structure):
     _fields_ = [("precision", C.c_int), ("pipeline", C.c_int), ("use_graph", C.c_int),
-                ("device", C.c_int), ("tile_option", C.c_int)]
+                ("device", C.c_int), ("tile_option", C.c_int), ("groups_per_cu", C.c_int)]
 
 
 class Diag(C.Structure):

@@ -203,6 +203,7 @@ NW_API void nw_opts_default(nw_opts* o) {
     o->use_graph = 1;
     o->device = 0;
     o->tile_option = -1;
+    o->groups_per_cu = 0;
 }
 
 NW_API int nw_dg_create(const nw_mesh* m, double c0, const nw_opts* opts, nw_dg** out) {

@@ -1063,13 +1063,18 @@ bool fused_supported(int n1, int prec) {
     return n1 >= 2 && n1 <= 9 && prec >= 0 && prec < kNumPrec;
 }
 
-int fused_resident(int n1, int prec, int option) {
+int fused_resident(int n1, int prec, int option, int max_per_cu) {
     ResidentFn fn = nullptr;
     if (prec == kFP64) fn = pick_resident_n1<double, double>(n1, option);
     else if (prec == kFP32) fn = pick_resident_n1<float, float>(n1, option);
     else if (prec == kFP32K64) fn = pick_resident_n1<float, double>(n1, option);
     else if (prec == kBF16X3 && n1 == 8) fn = &resident_of<8, 0, float, float, true>;
-    return fn ? fn() : 0;
+    const int all = fn ? fn() : 0;
+    if (all <= 0 || max_per_cu <= 0) return all;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) return 0;
+    return std::min(all / cus, max_per_cu) * cus;
 }
 
 hipError_t launch_fused(int n1, int prec, int option, const FusedArgs& a, hipStream_t s) {

@@ -77,7 +77,7 @@ int fused_epw(int n1, int option, int prec);
 // repeat a shape are not timed twice by the tuner).
 int fused_option_of(int n1, int option, int prec);
 // Workgroups of that shape resident on the current device at once (0: not built).
-int fused_resident(int n1, int prec, int option);
+int fused_resident(int n1, int prec, int option, int max_per_cu = 0);
 // Whether tile option `option` computes the volume contractions on the matrix cores.
 bool fused_matrix_volume(int n1, int option, int prec);
 bool fused_supported(int n1, int prec);

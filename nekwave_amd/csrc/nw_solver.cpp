@@ -280,6 +280,8 @@ int Solver::init(const Mesh* const* meshes, int count, double c0_, const nw_opts
     prec = o.precision;
     pipe = o.pipeline;
     use_graph = o.use_graph != 0;
+    NW_CHECK_ARG(o.groups_per_cu >= 0, "groups_per_cu must be >= 0");
+    groups_per_cu = o.groups_per_cu;
     autotune = (o.tile_option < 0) && pipe == NW_PIPE_FUSED;
     const int opt = o.tile_option < 0 ? 0 : std::min(o.tile_option, kNumTileOptions - 1);
     c0 = c0_;
@@ -411,7 +413,7 @@ int Solver::init(const Mesh* const* meshes, int count, double c0_, const nw_opts
         for (int opt = 0; opt < kNumTileOptions; ++opt)
             for (Segment& sg : seg) {
                 const int epw = fused_epw(sg.n1, opt, prec);
-                const int resident = fused_resident(sg.n1, prec, opt);
+                const int resident = fused_resident(sg.n1, prec, opt, groups_per_cu);
                 if (resident <= 0) {
                     set_error("no resident workgroups for the fused kernel");
                     return NW_ERR_HIP;
@@ -827,15 +829,16 @@ int Solver::build_graph(int parity) {
 // Tuned tile options and k policies kept for the life of the process, keyed by
 // what decides them: device, n1, precision, element count of the segment,
 // whether it is a single mesh (Morton walk) or a batch segment (storage order),
-// and whether the working set exceeds the Infinity Cache (non-temporal k a
-// candidate).  A later solver of the same shape takes the choice without timing
+// whether the working set exceeds the Infinity Cache (non-temporal k a
+// candidate) and the cap on workgroups per CU.  A later solver of the same shape takes the choice without timing
 // again.
 namespace {
 struct TuneKey {
-    int device, n1, prec, single, streamable;
+    int device, n1, prec, single, streamable, per_cu;
     long long nelt;
     bool operator<(const TuneKey& o) const {
         if (device != o.device) return device < o.device;
+        if (per_cu != o.per_cu) return per_cu < o.per_cu;
         if (n1 != o.n1) return n1 < o.n1;
         if (prec != o.prec) return prec < o.prec;
         if (single != o.single) return single < o.single;
@@ -860,7 +863,7 @@ int Solver::tune() {
     // Infinity Cache (below it the hint only costs)
     const int nstream = fused_beyond_infinity_cache(working_set_bytes()) ? 2 : 1;
     for (Segment& sg : seg) {
-        const TuneKey key{device, sg.n1, prec, inst.size() == 1 ? 1 : 0, nstream - 1, sg.nelt};
+        const TuneKey key{device, sg.n1, prec, inst.size() == 1 ? 1 : 0, nstream - 1, groups_per_cu, sg.nelt};
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
             auto it = g_tune.find(key);

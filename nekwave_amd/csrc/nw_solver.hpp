@@ -35,6 +35,7 @@ unsigned long long mesh_signature(const Mesh& m);
 
 struct Solver {
     int device = -1;
+    int groups_per_cu = 0;  // cap on fused workgroups per CU (0: as many as fit)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int prec = 0, pipe = 1;

@@ -110,6 +110,49 @@ def test_short_walk_segments_match_oracle():
     assert err <= 1e-5, err
 
 
+def test_groups_per_cu_cap_gives_the_same_bits():
+    """A cap on the fused launch's workgroups per CU (groups_per_cu) gives fewer,
+    longer workgroup sequences (other segment cuts of the x-chains, more carried
+    x faces) and the same bits: a 16 x 16 x 8 jittered periodic mesh at n1 = 8
+    in FP32 and FP64, and a batch of 40 cavities of orders 4 and 6, against the
+    uncapped runs."""
+    m = nw.Mesh(8)
+    m.createBoxMesh(16, 16, 8, 0, 1, 0, 1, 0, 1, True, True, True, jitter=0.1, seed=11)
+    rng = np.random.default_rng(11)
+    st = rng.standard_normal(6 * m.getTotalPoints())
+    dt = m.computeAutomaticDt(1.0, 0.4)
+    for prec in ("fp32", "fp64"):
+        runs = []
+        for cap in (0, 1, 2):
+            s = nw.DgSolver(precision=prec, pipeline="fused", use_graph=True, tile_option=0, groups_per_cu=cap)
+            s.initialize(m, 1.0)
+            s.uploadState(st)
+            s.run(dt, 3)
+            runs.append(s.downloadState())
+            s.finalize()
+        print("16x16x8 n1=8 %s: caps 1 and 2 equal uncapped: %s" % (prec, [np.array_equal(r, runs[0]) for r in runs[1:]]),
+              flush=True)
+        for r in runs[1:]:
+            assert np.array_equal(r, runs[0]), prec
+    meshes = []
+    for x in range(40):
+        n1 = 4 if x % 2 == 0 else 6
+        mm = nw.Mesh(n1)
+        mm.createBoxMesh(3, 3, 3, 0, 1, 0, 1 + 0.01 * x, 0, 1)
+        meshes.append(mm)
+    sts = [rng.standard_normal(6 * mm.getTotalPoints()) for mm in meshes]
+    runs = []
+    for cap in (0, 1):
+        b = nw.BatchSolver(precision="fp32", use_graph=True, tile_option=0, groups_per_cu=cap)
+        b.initialize(meshes, 1.0)
+        for x, v in enumerate(sts):
+            b.uploadState(x, v)
+        b.run(1e-3, 3)
+        runs.append([b.downloadState(x) for x in range(len(meshes))])
+        b.finalize()
+    assert all(np.array_equal(a, c) for a, c in zip(runs[0], runs[1]))
+
+
 def test_c2_bf16x3_vs_fp64_after_1000_steps(c2):
     """The split-BF16 mode's stated bound: relative L2 <= 1e-4 against the FP64
     path after 1000 steps of the 16^3 workload."""
