@@ -243,6 +243,9 @@ def test_set_segment_option_rejects_bad_arguments():
     with pytest.raises(nw.NekWaveError):
         p.setSegmentOption(0, 0, 0)
     p.finalize()
+    q = nw.DgSolver(precision="fp32", pipeline="fused", groups_per_cu=-1)
+    with pytest.raises(nw.NekWaveError):
+        q.initialize(m, c0)
 
 
 def test_segment_option_reports_matrix_volume():
