@@ -881,11 +881,15 @@ void k_fused_stage(FusedArgs A) {
                 un[c] = R(KR(sU[es][c][m]) + KR(A.b) * kn[c]);
             }
             // written through L2 (whole lines): the kernel boundary, which writes
-            // every dirty L2 line back for the other XCDs, has k's lines off its list
-            if (A.stream_k)
-                stk<kNT>(rK, kv, ks, kn);
-            else
-                stk<kSC1>(rK, kv, ks, kn);
+            // every dirty L2 line back for the other XCDs, has k's lines off its list.
+            // The last stage's k is dead (the next step's first stage has a = 0 and
+            // does not read k): not stored.
+            if (!A.last_stage) {
+                if (A.stream_k)
+                    stk<kNT>(rK, kv, ks, kn);
+                else
+                    stk<kSC1>(rK, kv, ks, kn);
+            }
             st6(rO, nd * UB, un);
         }
         geometry((t + 1) & 1, geo);  // the next group's record, staged in this iteration

@@ -48,6 +48,7 @@ struct FusedArgs {
     const double* dt;         // device scalar
     double a, b, c0;
     int first_stage;          // a == 0: k is not read
+    int last_stage;           // k is not stored: the next step's first stage (a == 0) does not read it
     const void* fw;           // per element face (nelt*6, segment-local), 4 R: flux weights
                               // (wEh, wEe, wHe, wHh), see face_flux_weights
     int affine;               // every element affine: records hold the geometry (launch_affine_records)

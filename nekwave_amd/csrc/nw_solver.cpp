@@ -611,6 +611,7 @@ int Solver::launch_segment(const Segment& sg, int s, int parity, int option, int
     a.b = kRkB[s];
     a.c0 = c0;
     a.first_stage = (s == 0);
+    a.last_stage = (s == 4);
     a.affine = sg.affine ? 1 : 0;
     a.stream_k = stream_k;
     a.fw = static_cast<const char*>(d_fw) + size_t(sg.elem_base) * 24 * (prec == kFP64 ? 8 : 4);
