@@ -37,11 +37,13 @@ int upload_gll_tables(int device) {
 }
 
 // LSRK45 coefficients, Carpenter & Kennedy (1994) (dg_solver.cpp:58-69).
-static const double kRkA[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
-                               -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
-static const double kRkB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
-                               1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
-                               2277821191437.0 / 14882151754819.0};
+constexpr int kStages = 5;  // LSRK45 stages per step
+static const double kRkA[kStages] = {0.0, -567301805773.0 / 1357537059087.0,
+                                     -2404267990393.0 / 2016746695238.0, -3550918686646.0 / 2091501179385.0,
+                                     -1275806237668.0 / 842570457699.0};
+static const double kRkB[kStages] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                                     1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                                     2277821191437.0 / 14882151754819.0};
 
 // Visiting order of the element groups (EPW consecutive elements each) of one
 // fused-kernel launch, as `nseq` workgroup sequences: workgroup W visits
@@ -611,7 +613,7 @@ int Solver::launch_segment(const Segment& sg, int s, int parity, int option, int
     a.b = kRkB[s];
     a.c0 = c0;
     a.first_stage = (s == 0);
-    a.last_stage = (s == 4);
+    a.last_stage = (s == kStages - 1);
     a.affine = sg.affine ? 1 : 0;
     a.stream_k = stream_k;
     a.fw = static_cast<const char*>(d_fw) + size_t(sg.elem_base) * 24 * (prec == kFP64 ? 8 : 4);
@@ -686,7 +688,7 @@ int Solver::enqueue_plain_stage(int s) {
 
 int Solver::enqueue_step(int parity) {
     int st;
-    for (int s = 0; s < 5; ++s) {
+    for (int s = 0; s < kStages; ++s) {
         if (pipe == NW_PIPE_FUSED) st = enqueue_fused_stage(s, (parity + s) & 1);
         else st = enqueue_plain_stage(s);
         if (st) return st;
@@ -831,8 +833,8 @@ int Solver::build_graph(int parity) {
 // what decides them: device, n1, precision, element count of the segment,
 // whether it is a single mesh (Morton walk) or a batch segment (storage order),
 // whether the working set exceeds the Infinity Cache (non-temporal k a
-// candidate) and the cap on workgroups per CU.  A later solver of the same shape takes the choice without timing
-// again.
+// candidate) and the cap on workgroups per CU.  A later solver of the same
+// shape takes the choice without timing again.
 namespace {
 struct TuneKey {
     int device, n1, prec, single, streamable, per_cu;
