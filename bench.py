@@ -49,7 +49,10 @@ SIZES = {"fp64": (8, 8), "fp32": (4, 4), "fp32_k64": (4, 8), "bf16x3": (4, 4)}
 def algorithmic_bytes_per_step(npts, nelt, s_u, s_k):
     """B_step = DOF (10 s_u + 9 s_k) + 5 nelt (24*8 + 2*8): per step, stage 1
     reads u, writes u and k; stages 2-5 read and write u and k; each stage reads
-    the element's 8 FP64 corners and its eps, mu."""
+    the element's 8 FP64 corners and its eps, mu.  This is the reference's
+    traffic, the figure every roofline fraction uses; the fused kernel's fifth
+    stage does not store k (nothing reads it), so the kernel itself moves
+    DOF (10 s_u + 8 s_k) plus the records."""
     return 6 * npts * (10 * s_u + 9 * s_k) + 5 * nelt * (24 * 8 + 2 * 8)
 
 
