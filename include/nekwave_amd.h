@@ -233,6 +233,15 @@ int nw_dg_segment_option(const nw_dg* s, int seg, int* option, int* matrix_volum
  * tuned: autotuning does not override the choice until nw_dg_tune is called.
  * Results do not depend on either choice. */
 int nw_dg_set_segment_option(nw_dg* s, int seg, int option, int stream_k);
+/* Extension, no counterpart in the reference: the walk direction of the fused
+ * stages.  mode 1: every other stage visits its element groups backwards, so it
+ * starts on the groups the stage before it wrote last (some of them still in
+ * the Infinity Cache); 0: every stage forwards; -1 (default): alternate where
+ * the stage's working set exceeds the 256 MiB Infinity Cache by at most 8x.
+ * nw_dg_walk reports whether the solver alternates.  Results do not depend
+ * on it. */
+int nw_dg_set_walk(nw_dg* s, int mode);
+int nw_dg_walk(const nw_dg* s, int* alternating);
 /* DgSolver::finalize (dg_solver.cpp:529-582) + destructor. */
 void nw_dg_destroy(nw_dg* s);
 

@@ -198,6 +198,13 @@ class DgSolver {
     void setSegmentOption(int seg, int option, int streamK = -1) {
         check(nw_dg_set_segment_option(s_, seg, option, streamK));
     }
+    // Extension: walk direction of the fused stages (nw_dg_set_walk / nw_dg_walk).
+    void setWalk(int mode) { check(nw_dg_set_walk(s_, mode)); }
+    bool alternatingWalk() const {
+        int a = 0;
+        check(nw_dg_walk(s_, &a));
+        return a != 0;
+    }
 
    private:
     nw_opts opts_;

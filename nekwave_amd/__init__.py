@@ -361,6 +361,18 @@ class DgSolver:
         (`segments()`); autotuning then keeps them until `tune()`."""
         check(lib().nw_dg_set_segment_option(self._handle(), int(seg), int(option), int(stream_k)))
 
+    def setWalk(self, mode):
+        """Walk direction of the fused stages (nw_dg_set_walk): 1 every other
+        stage backwards, 0 all forwards, -1 the default rule (alternate where
+        the working set exceeds the Infinity Cache by at most 8x)."""
+        check(lib().nw_dg_set_walk(self._handle(), int(mode)))
+
+    def alternatingWalk(self):
+        """Whether every other fused stage walks backwards (nw_dg_walk)."""
+        v = C.c_int()
+        check(lib().nw_dg_walk(self._handle(), C.byref(v)))
+        return bool(v.value)
+
     def segments(self):
         """Per n1 segment: n1, elements, elements per workgroup, tile option,
         whether it runs the volume contractions on the matrix cores and whether

@@ -93,6 +93,8 @@ SIGNATURES = {
     "nw_dg_segment_info": (i, [vp, i, ip, ip, ip]),
     "nw_dg_segment_option": (i, [vp, i, ip, ip, ip]),
     "nw_dg_set_segment_option": (i, [vp, i, i, i]),
+    "nw_dg_set_walk": (i, [vp, i]),
+    "nw_dg_walk": (i, [vp, ip]),
     "nw_launch_volume_curl": (i, [vp] * 18 + [d, i, i, vp]),
     "nw_launch_restrict_faces": (i, [vp, vp, vp, vp, i64, i64, vp]),
     "nw_launch_compute_flux": (i, [vp] * 9 + [d, i64, i64, vp, vp, vp, vp, i, i64, vp]),

@@ -303,6 +303,15 @@ NW_API int nw_dg_set_segment_option(nw_dg* s, int seg, int option, int stream_k)
     NW_CHECK_ARG(s, "null solver");
     return s->s.set_segment_option(seg, option, stream_k);
 }
+NW_API int nw_dg_set_walk(nw_dg* s, int mode) {
+    NW_CHECK_ARG(s, "null solver");
+    return s->s.set_walk(mode);
+}
+NW_API int nw_dg_walk(const nw_dg* s, int* alternating) {
+    NW_CHECK_ARG(s && alternating, "null argument");
+    *alternating = s->s.alternating_walk() ? 1 : 0;
+    return NW_OK;
+}
 NW_API int nw_dg_segment_info(const nw_dg* s, int seg, int* n1, int* nelt, int* epw) {
     NW_CHECK_ARG(s && seg >= 0 && seg < (int)s->s.seg.size(), "bad segment");
     const nw::Segment& g = s->s.seg[seg];

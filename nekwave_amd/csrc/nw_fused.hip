@@ -425,7 +425,9 @@ void k_fused_stage(FusedArgs A) {
     //   kGather: the gathered registers up[t];
     //   kSlot:   the neighbouring slot of this group (face 1 <-> slot + 1, face 3 <-> slot - 1);
     //   kCarry:  face 3 of slot 0, whose neighbour is the last slot of this
-    //            workgroup's previous group: its i = n1-1 plane, kept in sP3;
+    //            workgroup's previous group: its i = n1-1 plane, kept in sP3
+    //            (walking backwards: face 1 of the last slot, whose neighbour
+    //            is slot 0 of the previous group: its i = 0 plane);
     //   kMirror: PEC mirror face, no plus side.
     enum { kGather = 0, kSlot = 1, kCarry = 2, kMirror = 3 };
     struct Pre {
@@ -490,8 +492,13 @@ void k_fused_stage(FusedArgs A) {
                 const unsigned nb = unsigned(A.node_base), bx = unsigned(b.x);
                 if (f == 1 && efs + 1 < EPW && bx == nb + unsigned(clamp_e(g, efs + 1)) * NP) kind = kSlot;
                 if (f == 3 && efs > 0 && bx == nb + unsigned(clamp_e(g, efs - 1)) * NP + (N1 - 1)) kind = kSlot;
-                if (f == 3 && efs == 0 && gprev >= 0 && bx == nb + unsigned(clamp_e(gprev, EPW - 1)) * NP + (N1 - 1))
+                if (A.backwards) {
+                    if (f == 1 && efs == EPW - 1 && gprev >= 0 && bx == nb + unsigned(clamp_e(gprev, 0)) * NP)
+                        kind = kCarry;
+                } else if (f == 3 && efs == 0 && gprev >= 0 &&
+                           bx == nb + unsigned(clamp_e(gprev, EPW - 1)) * NP + (N1 - 1)) {
                     kind = kCarry;
+                }
             }
             // at n1 = 8 with one element per workgroup a wave holds one face: uniform
             if constexpr (SF) kind = __builtin_amdgcn_readfirstlane(kind);
@@ -865,8 +872,9 @@ void k_fused_stage(FusedArgs A) {
         // store on one path only would make the compiler's memory-counter
         // waits conservative and stall on the stores).
         // the last slot's i = n1-1 plane (pre-stage fields) for the next group's
-        // face 3; its readers in this iteration are past the barrier above
-        if (es == EPW - 1 && i == N1 - 1) {
+        // face 3 (walking backwards: slot 0's i = 0 plane for its face 1); its
+        // readers in this iteration are past the barrier above
+        if (A.backwards ? (es == 0 && i == 0) : (es == EPW - 1 && i == N1 - 1)) {
 #pragma unroll
             for (int c = 0; c < 6; ++c) sP3[j + N1 * k][c] = cur.u[c];
         }
@@ -1051,6 +1059,10 @@ int fused_option_of(int n1, int option, int prec) {
 }
 
 bool fused_beyond_infinity_cache(long long ws_bytes) { return ws_bytes > (256ll << 20); }
+
+bool fused_alternate_walk(long long ws_bytes) {
+    return fused_beyond_infinity_cache(ws_bytes) && ws_bytes <= 8 * (256ll << 20);
+}
 
 bool fused_stream_k(long long ws_bytes, int threads, int ksize) {
     return fused_beyond_infinity_cache(ws_bytes) && ((threads % 64) * ksize) % 128 == 0;

@@ -49,6 +49,8 @@ struct FusedArgs {
     double a, b, c0;
     int first_stage;          // a == 0: k is not read
     int last_stage;           // k is not stored: the next step's first stage (a == 0) does not read it
+    int backwards;            // gorder/ghead hold the backward walk: x-chains are walked towards -x,
+                              // the carried x face is face 1 of the last slot
     const void* fw;           // per element face (nelt*6, segment-local), 4 R: flux weights
                               // (wEh, wEe, wHe, wHh), see face_flux_weights
     int affine;               // every element affine: records hold the geometry (launch_affine_records)
@@ -64,6 +66,11 @@ struct FusedArgs {
 bool fused_stream_k(long long ws_bytes, int threads, int ksize);
 // Whether a stage's working set is larger than the 256 MiB Infinity Cache.
 bool fused_beyond_infinity_cache(long long ws_bytes);
+// Whether stages alternate the walk's direction: a stage that walks back over
+// the groups the stage before it wrote last finds some of them still in the
+// Infinity Cache -- worth it where the working set exceeds that cache but the
+// cache still holds a fair share (an eighth) of it.
+bool fused_alternate_walk(long long ws_bytes);
 // k chunks (6 x 64 values each) of one group of `threads` threads, and of a
 // segment of `nelt` elements under a tile option: a step's stages all use one
 // option, so the k buffer holds, per segment, the most any option needs.

@@ -411,7 +411,7 @@ int Solver::init(const Mesh* const* meshes, int count, double c0_, const nw_opts
             }
         }
         if ((st = upload_vec(&d_fnb, fnb))) return st;
-        std::vector<int> gorder, ghead;
+        std::vector<int> gorder, ghead, gorder_rev, ghead_rev;
         for (int opt = 0; opt < kNumTileOptions; ++opt)
             for (Segment& sg : seg) {
                 const int epw = fused_epw(sg.n1, opt, prec);
@@ -431,20 +431,33 @@ int Solver::init(const Mesh* const* meshes, int count, double c0_, const nw_opts
                 // (kFusedHead words; tl = -1: none), one scalar read for the kernel
                 const size_t go = static_cast<size_t>(sg.goff[opt]);
                 const int ns = sg.nseq[opt];
+                // the backward walk: every sequence reversed, so a stage starts on the
+                // groups the stage before it (walking forwards) wrote last
+                gorder_rev.resize(gorder.size());
+                for (int W = 0; W < ns; ++W)
+                    for (int p = seq[size_t(W)], q = seq[size_t(W) + 1] - 1; q >= seq[size_t(W)]; ++p, --q)
+                        gorder_rev[go + size_t(p)] = gorder[go + size_t(q)];
                 for (int hb = 0; hb < fused_grid(ns); ++hb) {
                     const int W = fused_sequence_of_block(hb, ns);
                     int head[kFusedHead] = {0, -1, 0, 0, 0, 0, 0, 0};
+                    int hrev[kFusedHead] = {0, -1, 0, 0, 0, 0, 0, 0};
                     if (W < ns) {
                         const int s0 = seq[size_t(W)], tl = seq[size_t(W) + 1] - s0 - 1;
-                        head[0] = s0;
-                        head[1] = tl;
-                        for (int t = 0; t < 3 && tl >= 0; ++t) head[2 + t] = gorder[go + size_t(s0 + std::min(t, tl))];
+                        head[0] = hrev[0] = s0;
+                        head[1] = hrev[1] = tl;
+                        for (int t = 0; t < 3 && tl >= 0; ++t) {
+                            head[2 + t] = gorder[go + size_t(s0 + std::min(t, tl))];
+                            hrev[2 + t] = gorder_rev[go + size_t(s0 + std::min(t, tl))];
+                        }
                     }
                     ghead.insert(ghead.end(), head, head + kFusedHead);
+                    ghead_rev.insert(ghead_rev.end(), hrev, hrev + kFusedHead);
                 }
             }
         if ((st = upload_vec(&d_gorder, gorder))) return st;
         if ((st = upload_vec(&d_ghead, ghead))) return st;
+        if ((st = upload_vec(&d_gorder_rev, gorder_rev))) return st;
+        if ((st = upload_vec(&d_ghead_rev, ghead_rev))) return st;
         // flux weights per element face from the two media (the plus side of a
         // mirror face is the element itself; of a cut face, its ghost medium)
         std::vector<double> w64(size_t(nelt) * 24);
@@ -588,7 +601,8 @@ int Solver::stream_of(const Segment& sg, int option) const {
     return fused_stream_k(working_set_bytes(), threads, ksize) ? 1 : 0;
 }
 
-int Solver::launch_segment(const Segment& sg, int s, int parity, int option, int stream_k, hipStream_t st) {
+int Solver::launch_segment(const Segment& sg, int s, int parity, int option, int stream_k, hipStream_t st,
+                           bool backwards) {
     FusedArgs a{};
     a.uin = d_u[parity];
     a.uout = d_u[1 - parity];
@@ -605,8 +619,9 @@ int Solver::launch_segment(const Segment& sg, int s, int parity, int option, int
     a.mat = d_mat;
     a.gmat = d_gmat;
     a.fnb = d_fnb + size_t(sg.elem_base) * 6;
-    a.gorder = d_gorder + sg.goff[option];
-    a.ghead = d_ghead + sg.soff[option];
+    a.gorder = (backwards ? d_gorder_rev : d_gorder) + sg.goff[option];
+    a.ghead = (backwards ? d_ghead_rev : d_ghead) + sg.soff[option];
+    a.backwards = backwards ? 1 : 0;
     a.nseq = sg.nseq[option];
     a.dt = d_dt;
     a.a = kRkA[s];
@@ -649,7 +664,11 @@ int Solver::enqueue_fused_stage(int s, int parity) {
     // would only compete for the same CUs.
     int st;
     for (size_t x = 0; x < seg.size(); ++x)
-        if ((st = launch_segment(seg[x], s, parity, seg[x].option, stream_of(seg[x], seg[x].option), stream))) return st;
+        // the ping-pong parity alternates from stage to stage, across steps too:
+        // with an alternating walk every other stage walks backwards
+        if ((st = launch_segment(seg[x], s, parity, seg[x].option, stream_of(seg[x], seg[x].option), stream,
+                                 parity != 0 && alternating_walk())))
+            return st;
     return NW_OK;
 }
 
@@ -947,6 +966,20 @@ int Solver::set_segment_option(int s, int option, int stream_k) {
     seg[size_t(s)].stream = stream_k;
     tuned = true;  // an explicit choice stands until tune() is called
     drop_graphs();
+    return NW_OK;
+}
+
+bool Solver::alternating_walk() const {
+    return pipe == NW_PIPE_FUSED && (walk_mode < 0 ? fused_alternate_walk(working_set_bytes()) : walk_mode == 1);
+}
+
+int Solver::set_walk(int mode) {
+    NW_CHECK_ARG(pipe == NW_PIPE_FUSED, "the walk belongs to the fused pipeline");
+    NW_CHECK_ARG(mode >= -1 && mode <= 1, "walk mode must be -1, 0 or 1");
+    NW_TRY_HIP(hipSetDevice(device));
+    NW_TRY_HIP(hipStreamSynchronize(stream));
+    walk_mode = mode;
+    drop_graphs();  // captured launches carry the previous direction
     return NW_OK;
 }
 

@@ -36,6 +36,7 @@ unsigned long long mesh_signature(const Mesh& m);
 struct Solver {
     int device = -1;
     int groups_per_cu = 0;  // cap on fused workgroups per CU (0: as many as fit)
+    int walk_mode = -1;     // walk direction of the fused stages (set_walk)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int prec = 0, pipe = 1;
@@ -69,6 +70,8 @@ struct Solver {
     double* d_rec = nullptr;  // per element: 24 trilinear coefficients, 1/eps, 1/mu
     int* d_gorder = nullptr;  // per segment and tile option: the order the fused kernel visits groups in
     int* d_ghead = nullptr;   // per segment and tile option: kFusedHead words per workgroup sequence
+    int* d_gorder_rev = nullptr;  // the same walks with every sequence visited backwards
+    int* d_ghead_rev = nullptr;
     // plain pipeline (reference residency, single mesh)
     double *d_state = nullptr, *d_kk = nullptr, *d_rhs = nullptr, *d_D = nullptr, *d_w3 = nullptr;
     double* d_met[13] = {};  // x y z jac rx sx tx ry sy ty rz sz tz
@@ -101,6 +104,8 @@ struct Solver {
     int probe_history(double* host, int max_records, int* nrecords);
     int tune();
     int set_segment_option(int s, int option, int stream_k);
+    int set_walk(int mode);  // -1 auto (fused_alternate_walk), 0 forwards, 1 alternating
+    bool alternating_walk() const;
     // non-temporal k accesses of segment sg under a tile option: the tuned
     // choice, else the default rule (fused_stream_k)
     int stream_of(const Segment& sg, int option) const;
@@ -112,7 +117,8 @@ struct Solver {
     int upload_vec(T** dst, const std::vector<T>& v);
     int init_plain(const Mesh& m);
     int set_dt(double dt);
-    int launch_segment(const Segment& sg, int s, int parity, int option, int stream_k, hipStream_t st);
+    int launch_segment(const Segment& sg, int s, int parity, int option, int stream_k, hipStream_t st,
+                       bool backwards = false);
     long long working_set_bytes() const;
     int enqueue_fused_stage(int s, int parity);
     int enqueue_plain_stage(int s);

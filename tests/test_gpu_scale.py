@@ -153,6 +153,43 @@ def test_groups_per_cu_cap_gives_the_same_bits():
     assert all(np.array_equal(a, c) for a, c in zip(runs[0], runs[1]))
 
 
+def test_backward_walks_give_the_same_bits():
+    """Every other fused stage walking its groups backwards (nw_dg_set_walk 1:
+    the carried x face is then face 1 of a group's last slot, from the i = 0
+    plane of the group visited before it) gives the bits of forward walks:
+    16 x 8 x 8 meshes at n1 = 8 and n1 = 6 (every tile option; one workgroup
+    per CU, so each walks several groups along its x-chains) and at n1 = 5 with
+    the halo stand-in on the x faces, FP32 and FP64, an odd number of steps
+    (both graphs)."""
+    rng = np.random.default_rng(13)
+    cases = []
+    for n1, opts, halo in ((8, (0, 1), False), (6, (0, 1, 2), False), (5, (0, 2), True)):
+        m = nw.Mesh(n1)
+        m.createBoxMesh(16, 8, 8, 0, 1, 0, 1, 0, 1, not halo, True, False, jitter=0.05, seed=13)
+        if halo:
+            m.selfHaloX()
+        cases.append((n1, opts, m))
+    for n1, opts, m in cases:
+        st = rng.standard_normal(6 * m.getTotalPoints())
+        dt = m.computeAutomaticDt(1.0, 0.4)
+        for prec in ("fp32", "fp64"):
+            for opt in opts:
+                runs = []
+                for mode in (0, 1):
+                    s = nw.DgSolver(precision=prec, pipeline="fused", use_graph=True, tile_option=opt,
+                                    groups_per_cu=1)
+                    s.initialize(m, 1.0)
+                    s.uploadState(st)
+                    s.setWalk(mode)
+                    assert s.alternatingWalk() == bool(mode)
+                    s.run(dt, 3)
+                    runs.append(s.downloadState())
+                    s.finalize()
+                print("n1=%d %s option %d: backward walks equal forward: %s"
+                      % (n1, prec, opt, np.array_equal(runs[0], runs[1])), flush=True)
+                assert np.array_equal(runs[0], runs[1]), (n1, prec, opt)
+
+
 def test_c2_bf16x3_vs_fp64_after_1000_steps(c2):
     """The split-BF16 mode's stated bound: relative L2 <= 1e-4 against the FP64
     path after 1000 steps of the 16^3 workload."""
