@@ -114,8 +114,8 @@ def test_groups_per_cu_cap_gives_the_same_bits():
     """A cap on the fused launch's workgroups per CU (groups_per_cu) gives fewer,
     longer workgroup sequences (other segment cuts of the x-chains, more carried
     x faces) and the same bits: a 16 x 16 x 8 jittered periodic mesh at n1 = 8
-    in FP32 and FP64, and a batch of 40 cavities of orders 4 and 6, against the
-    uncapped runs."""
+    in FP32 and FP64, and a batch of 40 cavities of orders 4 and 6 (capped, also
+    walking every other stage backwards), against the uncapped runs."""
     m = nw.Mesh(8)
     m.createBoxMesh(16, 16, 8, 0, 1, 0, 1, 0, 1, True, True, True, jitter=0.1, seed=11)
     rng = np.random.default_rng(11)
@@ -142,15 +142,17 @@ def test_groups_per_cu_cap_gives_the_same_bits():
         meshes.append(mm)
     sts = [rng.standard_normal(6 * mm.getTotalPoints()) for mm in meshes]
     runs = []
-    for cap in (0, 1):
+    for cap, walk in ((0, 0), (1, 0), (1, 1)):  # the last also walks every other stage backwards
         b = nw.BatchSolver(precision="fp32", use_graph=True, tile_option=0, groups_per_cu=cap)
         b.initialize(meshes, 1.0)
         for x, v in enumerate(sts):
             b.uploadState(x, v)
+        b.setWalk(walk)
         b.run(1e-3, 3)
         runs.append([b.downloadState(x) for x in range(len(meshes))])
         b.finalize()
-    assert all(np.array_equal(a, c) for a, c in zip(runs[0], runs[1]))
+    for r in runs[1:]:
+        assert all(np.array_equal(a, c) for a, c in zip(runs[0], r))
 
 
 def test_backward_walks_give_the_same_bits():
