@@ -260,7 +260,8 @@ static void StepLimit() {
 }
 
 // Extension API of the C++ header: every launch shape (tile option, LSRK
-// register policy) and the autotuner's choice give the bits of tile option 0.
+// register policy), the autotuner's choice and backward walks give the bits of
+// tile option 0.
 static void LaunchShapesGiveTheSameBits() {
     std::printf("[RUN] LaunchShapesGiveTheSameBits\n");
     const int N = 6;
@@ -273,7 +274,7 @@ static void LaunchShapesGiveTheSameBits() {
         for (int i = 0; i < npts; ++i) init[size_t(c) * npts + i] = std::sin(3.0 * x[i] + (c + 1) * y[i]) * std::cos(2.0 * z[i] - c);
     const double dt = 0.002;
     StateVector ref;
-    for (int run = 0; run < 7; ++run) {  // options 0..2 x k policy 0/1, then the tuner's pick
+    for (int run = 0; run < 8; ++run) {  // options 0..2 x k policy 0/1, the tuner's pick, backward walks
         nw_opts o;
         nw_opts_default(&o);
         o.precision = g_prec;
@@ -287,6 +288,9 @@ static void LaunchShapesGiveTheSameBits() {
             bool gotk = !sk;
             solver.segmentOption(0, &got, nullptr, &gotk);
             EXPECT_TRUE(got == opt && gotk == (sk != 0));
+        } else if (run == 7) {
+            solver.setWalk(1);
+            EXPECT_TRUE(solver.alternatingWalk());
         } else {
             solver.tune();
             int got = -1;
