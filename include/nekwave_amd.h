@@ -236,8 +236,9 @@ int nw_dg_set_segment_option(nw_dg* s, int seg, int option, int stream_k);
 /* Extension, no counterpart in the reference: the walk direction of the fused
  * stages.  mode 1: every other stage visits its element groups backwards, so it
  * starts on the groups the stage before it wrote last (some of them still in
- * the Infinity Cache); 0: every stage forwards; -1 (default): alternate where
- * the stage's working set exceeds the 256 MiB Infinity Cache by at most 8x.
+ * the Infinity Cache; a batch then also takes its segments in reverse order);
+ * 0: every stage forwards; -1 (default): alternate where the largest segment's
+ * working set exceeds the 256 MiB Infinity Cache by at most 8x.
  * nw_dg_walk reports whether the solver alternates.  Results do not depend
  * on it. */
 int nw_dg_set_walk(nw_dg* s, int mode);

@@ -363,8 +363,9 @@ class DgSolver:
 
     def setWalk(self, mode):
         """Walk direction of the fused stages (nw_dg_set_walk): 1 every other
-        stage backwards, 0 all forwards, -1 the default rule (alternate where
-        the working set exceeds the Infinity Cache by at most 8x)."""
+        stage backwards (a batch then also takes its segments in reverse
+        order), 0 all forwards, -1 the default rule (alternate where the largest
+        segment's working set exceeds the Infinity Cache by at most 8x)."""
         check(lib().nw_dg_set_walk(self._handle(), int(mode)))
 
     def alternatingWalk(self):

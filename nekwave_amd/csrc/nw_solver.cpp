@@ -663,12 +663,16 @@ int Solver::enqueue_fused_stage(int s, int parity) {
     // a persistent grid sized to fill the GPU, so concurrent segment launches
     // would only compete for the same CUs.
     int st;
-    for (size_t x = 0; x < seg.size(); ++x)
-        // the ping-pong parity alternates from stage to stage, across steps too:
-        // with an alternating walk every other stage walks backwards
-        if ((st = launch_segment(seg[x], s, parity, seg[x].option, stream_of(seg[x], seg[x].option), stream,
-                                 parity != 0 && alternating_walk())))
+    // the ping-pong parity alternates from stage to stage, across steps too:
+    // with an alternating walk every other stage walks backwards, and a batch
+    // then takes its segments in reverse order (the segment at the turn starts
+    // on its own last writes)
+    const bool back = parity != 0 && alternating_walk();
+    for (size_t i = 0; i < seg.size(); ++i) {
+        const size_t x = back ? seg.size() - 1 - i : i;
+        if ((st = launch_segment(seg[x], s, parity, seg[x].option, stream_of(seg[x], seg[x].option), stream, back)))
             return st;
+    }
     return NW_OK;
 }
 
@@ -969,8 +973,16 @@ int Solver::set_segment_option(int s, int option, int stream_k) {
     return NW_OK;
 }
 
+// Default rule on the largest segment's working set: segments run one after
+// another, and with the order reversed on backward stages the largest one (a
+// single mesh's only one) follows its own previous stage at the turn.
 bool Solver::alternating_walk() const {
-    return pipe == NW_PIPE_FUSED && (walk_mode < 0 ? fused_alternate_walk(working_set_bytes()) : walk_mode == 1);
+    if (pipe != NW_PIPE_FUSED) return false;
+    if (walk_mode >= 0) return walk_mode == 1;
+    long long most = 0;
+    for (const Segment& sg : seg)
+        most = std::max(most, (long long)sg.nelt * sg.n1 * sg.n1 * sg.n1);
+    return fused_alternate_walk(working_set_bytes() / std::max(1ll, npts) * most);
 }
 
 int Solver::set_walk(int mode) {
